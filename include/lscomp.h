@@ -1,0 +1,109 @@
+/*
+ * lscomp.h -- C ABI of the MI355X (gfx950) lsCOMP uint16 codec.
+ *
+ * Drop-in boundary for the reference's uint16 entry points
+ *   void cuLSZ_compression_uint16_bsize64(uint16_t* d_oriData, unsigned char* d_cmpBytes,
+ *        size_t* cmpSize, uint3 dims, uint4 quantBins, float poolingSH, cudaStream_t stream=0);
+ *   void cuLSZ_decompression_uint16_bsize64(uint16_t* d_decData, unsigned char* d_cmpBytes,
+ *        size_t cmpSize, uint3 dims, uint4 quantBins, float poolingSH, cudaStream_t stream=0);
+ * (lsCOMP/include/cuLSZ_entry.h:10-11, lsCOMP/src/cuLSZ_entry.cu:93-166).  The
+ * same-named C++ shims live in lscomp_culsz.h.
+ *
+ * Volume layout: dims.x frames (slowest) of dims.y rows of dims.z pixels
+ * (fastest), uint16, as in lsCOMP_xpcs.cpp:26.  Each frame is cut into 8x8
+ * blocks (zero padded at frame edges); every block gets a 1-byte header and
+ * a bit-plane payload.  Stream = [header of every block][zero pad to a multiple
+ * of 4][payloads in block order].  Dims and bins are not stored in the stream:
+ * decompression needs the same dims and bins; pooling_th only affects
+ * compression.
+ *
+ * Lossless: bins {1,1,1,1} and pooling_th = 1.  Lossy: bins x<=y<=z<=w > 0,
+ * pooling_th in [0,1] (a block is pooled when zero_count/64 > pooling_th).
+ *
+ * All pointers with a d_ prefix are device pointers on the current device.
+ * Calls are stream-ordered on `stream` (0 = null stream); the library keeps a
+ * small look-back scratch area per (device, stream), allocated on first use.
+ * Calls that overlap on one stream are ordered by the stream; calls on
+ * different streams run concurrently with separate scratch.
+ */
+#ifndef LSCOMP_H
+#define LSCOMP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lscomp_dim3 { uint32_t x, y, z; } lscomp_dim3;          /* uint3 of the reference */
+typedef struct lscomp_bins4 { uint32_t x, y, z, w; } lscomp_bins4;     /* uint4 of the reference */
+
+enum {
+    LSCOMP_OK = 0,
+    LSCOMP_ERR_INVALID_ARG = 1,  /* dims/bins/pooling out of domain, null or misaligned pointer */
+    LSCOMP_ERR_CAPACITY = 2,     /* output buffer too small; *out_bytes holds the required size */
+    LSCOMP_ERR_CORRUPT = 3,      /* stream shorter than its headers say, or a header rate > 16 */
+    LSCOMP_ERR_HIP = 4,          /* a HIP runtime call failed */
+    LSCOMP_ERR_TOO_LARGE = 5,    /* more than 2^31-1 blocks in one volume */
+    LSCOMP_ERR_INTERNAL = 6      /* look-back did not resolve (should not happen) */
+};
+
+/* Number of 8x8 blocks, x*ceil(y/8)*ceil(z/8) (cuLSZ_entry.cu:97-99). */
+size_t lscomp_num_blocks_u16(lscomp_dim3 dims);
+/* Header section size R = 4*ceil(blocks/4) (cuLSZ_kernel.cu:843). */
+size_t lscomp_header_bytes_u16(lscomp_dim3 dims);
+/* Worst-case stream size R + 128*blocks.  A buffer this large never overflows;
+ * the reference sizes it nbEle*2 (lsCOMP_xpcs.cpp:137), which can be short. */
+size_t lscomp_max_compressed_bytes_u16(lscomp_dim3 dims);
+
+/* Compress; replaces cuLSZ_compression_uint16_bsize64 (cuLSZ_entry.h:10).
+ * Blocks the host until the stream size is known and stores it in *out_bytes
+ * (R + sum of payloads; the reference reports sum + blocks, cuLSZ_entry.cu:125).
+ * If out_capacity is short, nothing past it is written, *out_bytes holds the
+ * required size and LSCOMP_ERR_CAPACITY is returned.  d_in and d_out must be
+ * 2- and 4-byte aligned; 16-byte aligned d_in with dims.z % 8 == 0 is fastest. */
+int lscomp_compress_u16(const uint16_t *d_in, uint8_t *d_out, size_t out_capacity,
+                        size_t *out_bytes, lscomp_dim3 dims, lscomp_bins4 bins,
+                        float pooling_th, hipStream_t stream);
+
+/* Stream-ordered compress: no host synchronization.  The stream size is
+ * written to device memory *d_out_bytes (8 bytes) when the kernel finishes;
+ * a size above out_capacity means the payload was not written. */
+int lscomp_compress_u16_async(const uint16_t *d_in, uint8_t *d_out, size_t out_capacity,
+                              uint64_t *d_out_bytes, lscomp_dim3 dims, lscomp_bins4 bins,
+                              float pooling_th, hipStream_t stream);
+
+/* Decompress; replaces cuLSZ_decompression_uint16_bsize64 (cuLSZ_entry.h:11).
+ * Waits for completion and returns LSCOMP_ERR_CORRUPT if the stream is
+ * inconsistent (those blocks decode to zeros).  Every element of the volume is
+ * written, rate-0 blocks as zeros. */
+int lscomp_decompress_u16(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes,
+                          lscomp_dim3 dims, lscomp_bins4 bins, hipStream_t stream);
+
+/* Stream-ordered decompress: no host synchronization; stream errors are
+ * sticky and reported by lscomp_stream_status(). */
+int lscomp_decompress_u16_async(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes,
+                                lscomp_dim3 dims, lscomp_bins4 bins, hipStream_t stream);
+
+/* Synchronizes `stream`, returns and clears errors raised by earlier async
+ * calls on it (LSCOMP_ERR_CORRUPT / LSCOMP_ERR_INTERNAL), or LSCOMP_OK. */
+int lscomp_stream_status(hipStream_t stream);
+
+/* Pre-allocates the scratch of `stream` for volumes up to `dims` so that
+ * later calls on it perform no allocation (e.g. before hipGraph capture). */
+int lscomp_reserve_u16(lscomp_dim3 dims, hipStream_t stream);
+
+/* Releases every scratch area of the current device (synchronizes). */
+void lscomp_release_all(void);
+
+const char *lscomp_status_string(int status);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int lscomp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSCOMP_H */

@@ -1,0 +1,369 @@
+// lscomp_api.cpp -- host side of the C ABI (include/lscomp.h).
+//
+// Replaces the reference's host entries cuLSZ_{compression,decompression}_
+// uint16_bsize64 (lsCOMP/src/cuLSZ_entry.cu:93-166).  Differences in how, not
+// what: validation happens in the library (the reference validates only in
+// its CLI, lsCOMP_xpcs.cpp:51-85); the look-back scratch is a persistent
+// per-(device, stream) area instead of three cudaMalloc/cudaMemset/cudaFree
+// per call (cuLSZ_entry.cu:109-130); errors are returned, not exit()ed.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "../../include/lscomp.h"
+#include "lscomp_internal.h"
+#include "lscomp_params.h"
+
+using namespace lsc;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Scratch areas: one per (device, stream).
+// ---------------------------------------------------------------------------
+struct Scratch {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    Ctrl *ctrl = nullptr;          // device, zero-initialised
+    uint64_t *status = nullptr;    // device, cap_tiles look-back words
+    uint64_t cap_tiles = 0;
+    uint64_t launches = 0;         // epochs used since the status words were cleared
+    Ctrl *host = nullptr;          // pinned readback of ctrl
+    std::mutex mu;
+};
+
+std::mutex g_mu;
+std::map<std::pair<int, hipStream_t>, Scratch *> g_scratch;
+
+// Any period below 2^26 launches keeps stale words of an equal epoch out.
+constexpr uint64_t kEpochClearPeriod = 1ull << 25;
+
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return LSCOMP_OK;
+    std::fprintf(stderr, "lscomp: %s failed: %s\n", what, hipGetErrorString(e));
+    return LSCOMP_ERR_HIP;
+}
+
+Scratch *get_scratch(hipStream_t stream)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto key = std::make_pair(dev, stream);
+    auto itr = g_scratch.find(key);
+    if (itr != g_scratch.end()) return itr->second;
+    Scratch *s = new Scratch;
+    s->device = dev;
+    s->stream = stream;
+    if (hipMalloc((void **)&s->ctrl, sizeof(Ctrl)) != hipSuccess ||
+        hipHostMalloc((void **)&s->host, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) {
+        if (s->ctrl) (void)hipFree(s->ctrl);
+        delete s;
+        return nullptr;
+    }
+    if (launch_fill_u64((uint64_t *)s->ctrl, sizeof(Ctrl) / 8, 0, stream) != hipSuccess) {
+        (void)hipFree(s->ctrl);
+        (void)hipHostFree(s->host);
+        delete s;
+        return nullptr;
+    }
+    g_scratch[key] = s;
+    return s;
+}
+
+// Caller holds s->mu.  Makes room for `tiles` look-back words.
+int ensure_tiles(Scratch *s, uint64_t tiles)
+{
+    if (tiles <= s->cap_tiles) return LSCOMP_OK;
+    uint64_t cap = s->cap_tiles ? s->cap_tiles : 1024;
+    while (cap < tiles) cap *= 2;
+    if (s->status) {
+        int st = hip_fail(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        if (st) return st;
+        (void)hipFree(s->status);
+        s->status = nullptr;
+        s->cap_tiles = 0;
+    }
+    int st = hip_fail(hipMalloc((void **)&s->status, cap * sizeof(uint64_t)), "hipMalloc");
+    if (st) return st;
+    st = hip_fail(launch_fill_u64(s->status, cap, 0, s->stream), "fill");
+    if (st) return st;
+    s->cap_tiles = cap;
+    s->launches = 0;
+    return LSCOMP_OK;
+}
+
+// Caller holds s->mu.  Called once per launch that uses the status words.
+int count_launch(Scratch *s)
+{
+    if (++s->launches >= kEpochClearPeriod) {
+        int st = hip_fail(launch_fill_u64(s->status, s->cap_tiles, 0, s->stream), "fill");
+        if (st) return st;
+        s->launches = 1;
+    }
+    return LSCOMP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Parameters.
+// ---------------------------------------------------------------------------
+bool bins_ok(lscomp_bins4 b)
+{
+    if (b.x == 0 || b.y == 0 || b.z == 0 || b.w == 0) return false;
+    return b.x <= b.y && b.y <= b.z && b.z <= b.w;
+}
+
+uint32_t align_class(const void *vol, uint32_t z)
+{
+    uintptr_t a = (uintptr_t)vol;
+    if (a % 16 == 0 && z % 8 == 0) return ALIGN16;
+    if (a % 4 == 0 && z % 2 == 0) return ALIGN4;
+    return ALIGN2;
+}
+
+uint64_t num_blocks(lscomp_dim3 d)
+{
+    return (uint64_t)d.x * (((uint64_t)d.y + 7) / 8) * (((uint64_t)d.z + 7) / 8);
+}
+
+// Fills the geometry part of an Item; returns a status.
+int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it)
+{
+    if (dims.x == 0 || dims.y == 0 || dims.z == 0) return LSCOMP_ERR_INVALID_ARG;
+    if (!bins_ok(bins)) return LSCOMP_ERR_INVALID_ARG;
+    if (!(p >= 0.0f && p <= 1.0f)) return LSCOMP_ERR_INVALID_ARG;
+    const uint64_t nb = num_blocks(dims);
+    if (nb > kMaxBlocksPerItem) return LSCOMP_ERR_TOO_LARGE;
+    std::memset(&it, 0, sizeof it);
+    it.x = dims.x;
+    it.y = dims.y;
+    it.z = dims.z;
+    it.nbz = (dims.z + 7) / 8;
+    it.per_frame = ((dims.y + 7) / 8) * it.nbz;
+    it.nblocks = (uint32_t)nb;
+    it.hdr_bytes = (uint32_t)((nb + 3) / 4 * 4);
+    it.first_tile = 0;
+    it.ntiles = (uint32_t)((nb + kTile - 1) / kTile);
+    it.div_frame = make_fastdiv(it.per_frame);
+    it.div_nbz = make_fastdiv(it.nbz);
+    const uint32_t b4[4] = {bins.x, bins.y, bins.z, bins.w};
+    it.dec = make_decide(b4, p);
+    return LSCOMP_OK;
+}
+
+LaunchCfg make_cfg(uint64_t total_tiles, bool compress)
+{
+    static int resident[2] = {0, 0};
+    int &r = resident[compress ? 1 : 0];
+    if (r == 0) r = max_resident_workgroups(compress);
+    LaunchCfg c;
+    c.total_tiles = (uint32_t)total_tiles;
+    uint64_t g = total_tiles;
+    if (r > 0 && g > (uint64_t)r) g = (uint64_t)r;
+    c.grid = (uint32_t)(g ? g : 1);
+    return c;
+}
+
+int read_ctrl(Scratch *s)
+{
+    int st = hip_fail(hipMemcpyAsync(s->host, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream),
+                      "hipMemcpyAsync");
+    if (st) return st;
+    return hip_fail(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+}
+
+// Returns the status for sticky error bits and clears them.
+int take_errors(Scratch *s)
+{
+    const uint32_t e = s->host->error;
+    if (!e) return LSCOMP_OK;
+    int st = hip_fail(launch_fill_u64((uint64_t *)&s->ctrl->error, 1, 0, s->stream), "fill");
+    if (st) return st;
+    st = hip_fail(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    if (st) return st;
+    if (e & ERRF_TIMEOUT) return LSCOMP_ERR_INTERNAL;
+    return LSCOMP_ERR_CORRUPT;
+}
+
+// Validates a compression call and fills its Item (no HIP calls).
+int prepare_compress(const uint16_t *d_in, uint8_t *d_out, size_t cap, lscomp_dim3 dims, lscomp_bins4 bins,
+                     float p, Item &it)
+{
+    int st = make_item(dims, bins, p, it);
+    if (st) return st;
+    if (!d_in || !d_out || ((uintptr_t)d_in & 1) || ((uintptr_t)d_out & 3)) return LSCOMP_ERR_INVALID_ARG;
+    if (cap < it.hdr_bytes) return LSCOMP_ERR_CAPACITY;
+    it.vol_in = d_in;
+    it.s_out = d_out;
+    it.s_len = cap;
+    it.align = align_class(d_in, dims.z);
+    return LSCOMP_OK;
+}
+
+int prepare_decompress(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                       lscomp_bins4 bins, Item &it)
+{
+    int st = make_item(dims, bins, 0.0f, it);
+    if (st) return st;
+    if (!d_in || !d_out || ((uintptr_t)d_out & 1) || ((uintptr_t)d_in & 3)) return LSCOMP_ERR_INVALID_ARG;
+    if (in_bytes < it.hdr_bytes) return LSCOMP_ERR_CORRUPT;
+    it.vol_out = d_out;
+    it.s_in = d_in;
+    it.s_len = in_bytes;
+    it.align = align_class(d_out, dims.z);
+    return LSCOMP_OK;
+}
+
+// Caller holds s->mu.
+int launch_item(Scratch *s, Item &it, bool compress)
+{
+    int st = ensure_tiles(s, it.ntiles);
+    if (st) return st;
+    st = count_launch(s);
+    if (st) return st;
+    if (compress) {
+        if (!it.result) it.result = &s->ctrl->result;
+        return hip_fail(launch_compress(it, nullptr, 1, make_cfg(it.ntiles, true), s->ctrl, s->status, s->stream),
+                        "compress launch");
+    }
+    return hip_fail(launch_decompress(it, nullptr, 1, make_cfg(it.ntiles, false), s->ctrl, s->status, s->stream),
+                    "decompress launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t lscomp_num_blocks_u16(lscomp_dim3 dims) { return (size_t)num_blocks(dims); }
+
+size_t lscomp_header_bytes_u16(lscomp_dim3 dims) { return (size_t)((num_blocks(dims) + 3) / 4 * 4); }
+
+size_t lscomp_max_compressed_bytes_u16(lscomp_dim3 dims)
+{
+    const uint64_t nb = num_blocks(dims);
+    return (size_t)((nb + 3) / 4 * 4 + 128 * nb);
+}
+
+int lscomp_compress_u16_async(const uint16_t *d_in, uint8_t *d_out, size_t out_capacity, uint64_t *d_out_bytes,
+                              lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream)
+{
+    if (!d_out_bytes || ((uintptr_t)d_out_bytes & 7)) return LSCOMP_ERR_INVALID_ARG;
+    Item it;
+    int st = prepare_compress(d_in, d_out, out_capacity, dims, bins, pooling_th, it);
+    if (st) return st;
+    it.result = d_out_bytes;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return launch_item(s, it, true);
+}
+
+int lscomp_compress_u16(const uint16_t *d_in, uint8_t *d_out, size_t out_capacity, size_t *out_bytes,
+                        lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream)
+{
+    if (!out_bytes) return LSCOMP_ERR_INVALID_ARG;
+    Item it;
+    int st = prepare_compress(d_in, d_out, out_capacity, dims, bins, pooling_th, it);
+    if (st == LSCOMP_ERR_CAPACITY) *out_bytes = lscomp_header_bytes_u16(dims);
+    if (st) return st;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    st = launch_item(s, it, true);
+    if (st) return st;
+    st = read_ctrl(s);
+    if (st) return st;
+    *out_bytes = (size_t)s->host->result;
+    st = take_errors(s);
+    if (st) return st;
+    return *out_bytes > out_capacity ? LSCOMP_ERR_CAPACITY : LSCOMP_OK;
+}
+
+int lscomp_decompress_u16_async(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                                lscomp_bins4 bins, hipStream_t stream)
+{
+    Item it;
+    int st = prepare_decompress(d_out, d_in, in_bytes, dims, bins, it);
+    if (st) return st;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return launch_item(s, it, false);
+}
+
+int lscomp_decompress_u16(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                          lscomp_bins4 bins, hipStream_t stream)
+{
+    Item it;
+    int st = prepare_decompress(d_out, d_in, in_bytes, dims, bins, it);
+    if (st) return st;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    st = launch_item(s, it, false);
+    if (st) return st;
+    st = read_ctrl(s);
+    if (st) return st;
+    return take_errors(s);
+}
+
+int lscomp_stream_status(hipStream_t stream)
+{
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    int st = read_ctrl(s);
+    if (st) return st;
+    return take_errors(s);
+}
+
+int lscomp_reserve_u16(lscomp_dim3 dims, hipStream_t stream)
+{
+    Item it;
+    int st = make_item(dims, lscomp_bins4{1, 1, 1, 1}, 1.0f, it);
+    if (st) return st;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return ensure_tiles(s, it.ntiles);
+}
+
+void lscomp_release_all(void)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto itr = g_scratch.begin(); itr != g_scratch.end();) {
+        Scratch *s = itr->second;
+        if (s->device != dev) { ++itr; continue; }
+        (void)hipStreamSynchronize(s->stream);
+        if (s->status) (void)hipFree(s->status);
+        if (s->ctrl) (void)hipFree(s->ctrl);
+        if (s->host) (void)hipHostFree(s->host);
+        delete s;
+        itr = g_scratch.erase(itr);
+    }
+}
+
+const char *lscomp_status_string(int status)
+{
+    switch (status) {
+    case LSCOMP_OK: return "ok";
+    case LSCOMP_ERR_INVALID_ARG: return "invalid argument";
+    case LSCOMP_ERR_CAPACITY: return "output buffer too small";
+    case LSCOMP_ERR_CORRUPT: return "corrupt or truncated stream";
+    case LSCOMP_ERR_HIP: return "HIP runtime error";
+    case LSCOMP_ERR_TOO_LARGE: return "volume has more than 2^31-1 blocks";
+    case LSCOMP_ERR_INTERNAL: return "internal error (look-back timeout)";
+    default: return "unknown status";
+    }
+}
+
+int lscomp_version(void) { return 100; }
+
+}  // extern "C"

@@ -1,0 +1,317 @@
+// lscomp_block.h -- per-block arithmetic of the lsCOMP uint16 codec, written for
+// a lane that owns one whole 8x8 block in 32 packed registers.
+//
+// Register layout: D[4*r + c] = v[r][2c] | v[r][2c+1] << 16 (row r of the
+// block, column k = 2c or 2c+1).  Everything here is integer and bit-exact.
+//
+// The functions compile both for gfx950 (hipcc) and for the host (clang++),
+// so the CPU test suite checks this exact code against the oracle before any
+// GPU run; only v_perm_b32 and v_alignbit_b32 have host emulations.
+//
+// Semantics follow the reference kernel lsCOMP/src/cuLSZ_kernel.cu:
+//   statistics 898-917, pooling/bin choice 919-935, rate/header 937-957,
+//   plain packing 1169-1277, pooled packing 1102-1168, unpacking 1467-1627.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define LSC_HD __host__ __device__ __forceinline__
+#else
+#define LSC_HD inline __attribute__((always_inline))
+#endif
+
+namespace lsc {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+LSC_HD u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+LSC_HD uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// v_perm_b32: result byte i = byte sel.byte[i] of the 8-byte value {hi:lo}
+// (selector 0..3 picks from lo, 4..7 from hi).  Only selectors 0..7 are used.
+LSC_HD uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+    uint64_t v = ((uint64_t)hi << 32) | lo;
+    uint32_t r = 0;
+    for (int i = 0; i < 4; i++) {
+        uint32_t s = (sel >> (8 * i)) & 7u;
+        r |= (uint32_t)((v >> (8 * s)) & 0xffu) << (8 * i);
+    }
+    return r;
+#endif
+}
+
+// Rotate right by s (mod 32): v_alignbit_b32(x, x, s).
+LSC_HD uint32_t rotr(uint32_t x, uint32_t s)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(x, x, s);
+#else
+    s &= 31u;
+    return s ? (x >> s) | (x << (32u - s)) : x;
+#endif
+}
+
+LSC_HD uint32_t bit_width(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v) : 0u; }
+
+// ---------------------------------------------------------------------------
+// Division by a per-block bin: q = floor(v / d) for v < 2^16, any d >= 1,
+// as q = trunc((float)v * f) with f = fl((1 + 2^-20) / d).
+// Proof: f*v = (v/d)(1 + 2^-20)(1 + e), |e| <= 2^-23, so the product lies in
+// [v/d, v/d + (v/d) 2^-19).  v/d <= q + 1 - 1/d and (v/d) 2^-19 < 1/d for
+// v < 2^19, hence q <= f*v < q + 1.  Checked exhaustively on the CPU.
+// ---------------------------------------------------------------------------
+LSC_HD uint32_t quant(uint32_t v, float f) { return (uint32_t)((float)v * f); }
+
+// Fast u32 division by an invariant divisor (round-up multiplier method,
+// Granlund & Montgomery 1994): q = (umulhi(n, m) + n) >> s.
+struct FastDiv {
+    uint32_t d, m, s, pad;
+};
+LSC_HD uint32_t fdiv(uint32_t n, FastDiv fd)
+{
+    uint32_t hi = (uint32_t)(((uint64_t)n * fd.m) >> 32);
+    return (uint32_t)(((uint64_t)hi + n) >> fd.s);
+}
+
+// ---------------------------------------------------------------------------
+// Block statistics and decision.
+// ---------------------------------------------------------------------------
+struct Decide {
+    uint32_t b[4];     // quantization bins
+    float f[4];        // fl((1 + 2^-20) / b[k])
+    uint32_t zth;      // pool iff zero_count >= zth (65: never)
+    uint32_t flags;    // DEC_*
+};
+enum : uint32_t {
+    DEC_LOSSLESS = 1u,  // all bins 1 and no pooling: q == v
+    DEC_POOL = 2u,      // pooling possible (zth <= 64)
+    DEC_BINSEL = 4u,    // bin choice depends on the data (b0 != b3, b0 < 2^16)
+};
+
+// Header byte of a block: (pool << 7) | (bin_choice << 5) | rate.
+LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
+{
+    // max of all 64 values (packed 16-bit max)
+    u16x2 mx = as_u16x2(D[0]);
+#pragma unroll
+    for (int i = 1; i < 32; i++) mx = __builtin_elementwise_max(mx, as_u16x2(D[i]));
+    uint32_t m1 = mx.x > mx.y ? mx.x : mx.y;
+    if (p.flags & DEC_LOSSLESS) return (3u << 5) | bit_width(m1);
+
+    uint32_t pool = 0, m2 = 0;
+    if (p.flags & DEC_POOL) {
+        // nonzero count (min(v,1) summed per half) and max pair sum
+        u16x2 nz = {0, 0};
+        uint32_t s2 = 0;
+        const u16x2 one = {1, 1};
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            u16x2 v = as_u16x2(D[i]);
+            nz += __builtin_elementwise_min(v, one);
+            uint32_t s = (uint32_t)v.x + (uint32_t)v.y;
+            s2 = s > s2 ? s : s2;
+        }
+        uint32_t zc = 64u - ((uint32_t)nz.x + (uint32_t)nz.y);
+        pool = zc >= p.zth;
+        m2 = s2 >> 1;   // max floor((a+b)/2) == floor(max(a+b)/2)
+    }
+
+    uint32_t bc = 3;
+    if (p.flags & DEC_BINSEL) {
+        // bin choice: bc = #{k in 1..3 : m0 >= b[k]}, m0 = min{v : v >= b0}
+        // (no value in [b0,b1) <=> m0 >= b1; bins are sorted).  Computed as
+        // min over (v - b0) mod 2^16, which is < 2^16 - b0 iff some v >= b0.
+        const u16x2 b0 = {(unsigned short)p.b[0], (unsigned short)p.b[0]};
+        u16x2 mt = as_u16x2(D[0]) - b0;
+#pragma unroll
+        for (int i = 1; i < 32; i++) mt = __builtin_elementwise_min(mt, as_u16x2(D[i]) - b0);
+        uint32_t t = mt.x < mt.y ? mt.x : mt.y;
+        uint32_t m0 = (t <= 65535u - p.b[0]) ? t + p.b[0] : 0xffffffffu;
+        bc = (uint32_t)(m0 >= p.b[1]) + (uint32_t)(m0 >= p.b[2]) + (uint32_t)(m0 >= p.b[3]);
+    }
+    float f = bc == 0 ? p.f[0] : bc == 1 ? p.f[1] : bc == 2 ? p.f[2] : p.f[3];
+    uint32_t maxq = quant(pool ? m2 : m1, f);
+    return (pool << 7) | (bc << 5) | bit_width(maxq);
+}
+
+LSC_HD uint32_t header_payload_bytes(uint32_t hdr)
+{
+    return (hdr & 0x1fu) * ((hdr & 0x80u) ? 4u : 8u);
+}
+
+// ---------------------------------------------------------------------------
+// Plain blocks.  Column words: CL[k] (rows 0..3) and CL[8+k] (rows 4..7) hold
+// the low bytes of q[row][k], one row per byte; CH the high bytes.  Plane i
+// (8 bytes, byte r = row r, bit 7-k = bit i of q[r][k]) is then
+//   OR_k rotr(C[k], i + k - 7) & (0x01010101 << (7 - k))   per 4-row half.
+// ---------------------------------------------------------------------------
+LSC_HD void columns_from_rows(const uint32_t D[32], uint32_t CL[16], uint32_t CH[16])
+{
+#pragma unroll
+    for (int h = 0; h < 2; h++)           // row half: rows 4h..4h+3
+#pragma unroll
+        for (int c = 0; c < 4; c++) {     // column pair (2c, 2c+1)
+            const uint32_t d0 = D[4 * (4 * h + 0) + c], d1 = D[4 * (4 * h + 1) + c];
+            const uint32_t d2 = D[4 * (4 * h + 2) + c], d3 = D[4 * (4 * h + 3) + c];
+            // low bytes: (row0 k, row1 k, row0 k+1, row1 k+1)
+            uint32_t l01 = perm(d1, d0, 0x06020400u), l23 = perm(d3, d2, 0x06020400u);
+            uint32_t h01 = perm(d1, d0, 0x07030501u), h23 = perm(d3, d2, 0x07030501u);
+            CL[8 * h + 2 * c] = perm(l23, l01, 0x05040100u);
+            CL[8 * h + 2 * c + 1] = perm(l23, l01, 0x07060302u);
+            CH[8 * h + 2 * c] = perm(h23, h01, 0x05040100u);
+            CH[8 * h + 2 * c + 1] = perm(h23, h01, 0x07060302u);
+        }
+}
+
+LSC_HD void rows_from_columns(const uint32_t CL[16], const uint32_t CH[16], uint32_t D[32])
+{
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t le = CL[8 * h + 2 * c], lo = CL[8 * h + 2 * c + 1];
+            const uint32_t he = CH[8 * h + 2 * c], ho = CH[8 * h + 2 * c + 1];
+            uint32_t e01 = perm(he, le, 0x05010400u), e23 = perm(he, le, 0x07030602u);
+            uint32_t o01 = perm(ho, lo, 0x05010400u), o23 = perm(ho, lo, 0x07030602u);
+            D[4 * (4 * h + 0) + c] = perm(o01, e01, 0x05040100u);
+            D[4 * (4 * h + 1) + c] = perm(o01, e01, 0x07060302u);
+            D[4 * (4 * h + 2) + c] = perm(o23, e23, 0x05040100u);
+            D[4 * (4 * h + 3) + c] = perm(o23, e23, 0x07060302u);
+        }
+}
+
+// bit plane i (0..7) of 8 column words -> 4 bytes
+LSC_HD uint32_t plane_from_columns(const uint32_t *C, uint32_t i)
+{
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        acc |= rotr(C[k], i + (uint32_t)k - 7u) & (0x01010101u << (7 - k));
+    return acc;
+}
+
+// OR bit plane i (0..7) back into 8 column words
+LSC_HD void columns_add_plane(uint32_t *C, uint32_t P, uint32_t i)
+{
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        C[k] |= rotr(P, 7u - (uint32_t)k - i) & (0x01010101u << i);
+}
+
+// Quantize 64 values in place.
+LSC_HD void quantize_plain(uint32_t D[32], float f)
+{
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        uint32_t lo = quant(D[i] & 0xffffu, f), hi = quant(D[i] >> 16, f);
+        D[i] = lo | (hi << 16);
+    }
+}
+
+// Pooled: Q[t] = floor((v[2t] + v[2t+1]) / 2) / bin, t = 4r + c (pair c of row r).
+LSC_HD void quantize_pooled(const uint32_t D[32], float f, bool identity, uint32_t Q[32])
+{
+#pragma unroll
+    for (int t = 0; t < 32; t++) {
+        uint32_t a = ((D[t] & 0xffffu) + (D[t] >> 16)) >> 1;
+        Q[t] = identity ? a : quant(a, f);
+    }
+}
+
+// Pooled column words: CP[u] byte m = low byte of Q[8m+u], CPH the high byte.
+LSC_HD void pooled_columns(const uint32_t Q[32], uint32_t CP[8], uint32_t CPH[8])
+{
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        uint32_t a = Q[u], b = Q[8 + u], c = Q[16 + u], d = Q[24 + u];
+        uint32_t ab = perm(b, a, 0x05010400u);   // (a.lo, b.lo, a.hi, b.hi)
+        uint32_t cd = perm(d, c, 0x05010400u);   // (c.lo, d.lo, c.hi, d.hi)
+        CP[u] = perm(cd, ab, 0x05040100u);       // (a.lo, b.lo, c.lo, d.lo)
+        CPH[u] = perm(cd, ab, 0x07060302u);      // (a.hi, b.hi, c.hi, d.hi)
+    }
+}
+
+// Encode the payload of one block into words out[0..]; returns the number of
+// 32-bit words (2*rate plain, rate pooled).  hdr from block_header().
+// Quantizes D in place for plain blocks.
+LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint32_t *out)
+{
+    const uint32_t rate = hdr & 0x1fu;
+    if (rate == 0) return 0;
+    const uint32_t bc = (hdr >> 5) & 3u;
+    const bool identity = p.b[bc] == 1u;
+    const float f = bc == 0 ? p.f[0] : bc == 1 ? p.f[1] : bc == 2 ? p.f[2] : p.f[3];
+    if (hdr & 0x80u) {
+        uint32_t Q[32], CP[8], CPH[8];
+        quantize_pooled(D, f, identity, Q);
+        pooled_columns(Q, CP, CPH);
+        const uint32_t nlo = rate < 8 ? rate : 8u;
+        for (uint32_t i = 0; i < nlo; i++) out[i] = plane_from_columns(CP, i);
+        for (uint32_t i = 8; i < rate; i++) out[i] = plane_from_columns(CPH, i - 8);
+        return rate;
+    }
+    if (!identity) quantize_plain(D, f);
+    uint32_t CL[16], CH[16];
+    columns_from_rows(D, CL, CH);
+    const uint32_t nlo = rate < 8 ? rate : 8u;
+    for (uint32_t i = 0; i < nlo; i++) {
+        out[2 * i] = plane_from_columns(CL, i);
+        out[2 * i + 1] = plane_from_columns(CL + 8, i);
+    }
+    for (uint32_t i = 8; i < rate; i++) {
+        out[2 * i] = plane_from_columns(CH, i - 8);
+        out[2 * i + 1] = plane_from_columns(CH + 8, i - 8);
+    }
+    return 2 * rate;
+}
+
+// Decode one block from its payload words (in[0..]) into D.  rate <= 16.
+// De-quantization is modulo 2^16 like the reference's uint16 store.
+LSC_HD void decode_block(const uint32_t *in, uint32_t hdr, const uint32_t bins[4], uint32_t D[32])
+{
+    const uint32_t rate = hdr & 0x1fu;
+    const uint32_t bin = bins[(hdr >> 5) & 3u];
+    if (hdr & 0x80u) {
+        uint32_t CP[8] = {0, 0, 0, 0, 0, 0, 0, 0}, CPH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const uint32_t nlo = rate < 8 ? rate : 8u;
+        for (uint32_t i = 0; i < nlo; i++) columns_add_plane(CP, in[i], i);
+        for (uint32_t i = 8; i < rate; i++) columns_add_plane(CPH, in[i], i - 8);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            // Q[8m+u] = CP[u].byte m | CPH[u].byte m << 8
+            uint32_t q01 = perm(CPH[u], CP[u], 0x05010400u);   // Q[u], Q[8+u]
+            uint32_t q23 = perm(CPH[u], CP[u], 0x07030602u);   // Q[16+u], Q[24+u]
+            uint32_t qs[4] = {q01 & 0xffffu, q01 >> 16, q23 & 0xffffu, q23 >> 16};
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                uint32_t v = (qs[m] * bin) & 0xffffu;
+                D[8 * m + u] = v | (v << 16);
+            }
+        }
+        return;
+    }
+    uint32_t CL[16], CH[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) { CL[j] = 0; CH[j] = 0; }
+    const uint32_t nlo = rate < 8 ? rate : 8u;
+    for (uint32_t i = 0; i < nlo; i++) {
+        columns_add_plane(CL, in[2 * i], i);
+        columns_add_plane(CL + 8, in[2 * i + 1], i);
+    }
+    for (uint32_t i = 8; i < rate; i++) {
+        columns_add_plane(CH, in[2 * i], i - 8);
+        columns_add_plane(CH + 8, in[2 * i + 1], i - 8);
+    }
+    rows_from_columns(CL, CH, D);
+    if (bin != 1u) {
+        const u16x2 bb = {(unsigned short)bin, (unsigned short)bin};
+#pragma unroll
+        for (int i = 0; i < 32; i++) D[i] = as_u32(as_u16x2(D[i]) * bb);
+    }
+}
+
+}  // namespace lsc

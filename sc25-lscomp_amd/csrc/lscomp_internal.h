@@ -1,0 +1,71 @@
+// lscomp_internal.h -- types shared by the gfx950 kernels and the host API.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "lscomp_block.h"
+
+namespace lsc {
+
+constexpr int kTile = 256;               // blocks per tile == threads per workgroup
+constexpr int kWaves = kTile / 64;
+constexpr uint32_t kMaxBlocksPerItem = 0x7fffffffu;   // prefix values fit 36-bit words
+
+// Alignment class of a volume: how a row of one block (8 x u16) may be loaded.
+enum : uint32_t { ALIGN2 = 0, ALIGN4 = 1, ALIGN16 = 2 };
+
+// Launch-state block of a scratch area.  All fields start at zero; the last
+// workgroup of every launch resets the counters and advances the epoch, so a
+// launch needs no host-side clearing (and replays inside a hipGraph).
+struct Ctrl {
+    uint32_t tile_ctr;   // dynamic tile ids
+    uint32_t done_ctr;   // workgroups finished
+    uint32_t epoch;      // tags look-back words of the current launch
+    uint32_t pad0;
+    uint32_t error;      // sticky ERRF_* bits, cleared by the host
+    uint32_t pad1;
+    uint64_t result;     // stream bytes of a single-item compression
+    uint64_t pad2[4];
+};
+static_assert(sizeof(Ctrl) == 64, "Ctrl is one 64-byte record");
+enum : uint32_t { ERRF_CORRUPT = 1u, ERRF_TIMEOUT = 2u };
+
+// Look-back word: epoch (26 bits) | flag (2 bits) | value in 4-byte words (36 bits).
+constexpr uint32_t kEpochMask = (1u << 26) - 1u;
+constexpr uint64_t kValMask = (1ull << 36) - 1ull;
+enum : uint32_t { LB_EMPTY = 0, LB_AGG = 1, LB_INCL = 2 };
+
+// One independent stream: a volume of x frames of y rows of z pixels.
+struct Item {
+    const uint16_t *vol_in;   // compress: source volume
+    uint16_t *vol_out;        // decompress: destination volume
+    const uint8_t *s_in;      // decompress: source stream
+    uint8_t *s_out;           // compress: destination stream
+    uint64_t s_len;           // compress: capacity; decompress: stream bytes
+    uint64_t *result;         // compress: required stream bytes (or null)
+    uint32_t x, y, z, nbz;    // dims; nbz = ceil(z/8)
+    uint32_t per_frame;       // ceil(y/8) * ceil(z/8)
+    uint32_t nblocks;         // x * per_frame
+    uint32_t hdr_bytes;       // R = 4 * ceil(nblocks/4)
+    uint32_t first_tile;      // first global tile of this item
+    uint32_t ntiles;          // ceil(nblocks / kTile)
+    uint32_t align;           // ALIGN*
+    uint32_t pad0, pad1;
+    FastDiv div_frame, div_nbz;
+    Decide dec;
+};
+
+struct LaunchCfg {
+    uint32_t grid;            // workgroups
+    uint32_t total_tiles;
+};
+
+// Kernel launchers (lscomp_kernels.hip).  `items` is a device array of
+// nitems descriptors sorted by first_tile, or null for the single item.
+hipError_t launch_compress(const Item &single, const Item *items, uint32_t nitems,
+                           LaunchCfg cfg, Ctrl *ctrl, uint64_t *status, hipStream_t stream);
+hipError_t launch_decompress(const Item &single, const Item *items, uint32_t nitems,
+                             LaunchCfg cfg, Ctrl *ctrl, uint64_t *status, hipStream_t stream);
+hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t value, hipStream_t stream);
+int max_resident_workgroups(bool compress);
+
+}  // namespace lsc

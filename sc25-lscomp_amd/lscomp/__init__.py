@@ -1,0 +1,201 @@
+"""Python binding of the gfx950 lsCOMP uint16 codec (liblscomp.so, C ABI in include/lscomp.h).
+
+Host-side mirror of the reference's compressor interface
+(lsCOMP/include/cuLSZ_entry.h:10-11): ``compress`` / ``decompress`` take a
+device volume of ``dims = (x, y, z)`` (x frames, z fastest), four
+quantization ``bins`` (x <= y <= z <= w) and a ``pooling`` threshold in [0, 1].
+Lossless is ``bins=(1, 1, 1, 1), pooling=1.0``.
+
+Device memory and streams come from PyTorch (plumbing only); every byte of
+compression and decompression is computed by the HIP kernels in the shared
+library.  There is no CPU fallback: if the library is missing, importing this
+module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblscomp.so")
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_CAPACITY = 2
+ERR_CORRUPT = 3
+ERR_HIP = 4
+ERR_TOO_LARGE = 5
+ERR_INTERNAL = 6
+
+LOSSLESS_BINS = (1, 1, 1, 1)
+
+
+class LscompError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = _lib.lscomp_status_string(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{what}: {msg} (status {status})" if what else msg)
+
+
+class Dim3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32)]
+
+
+class Bins4(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32),
+                ("w", ctypes.c_uint32)]
+
+
+# Every symbol include/lscomp.h declares.
+EXPORTS = (
+    "lscomp_num_blocks_u16", "lscomp_header_bytes_u16", "lscomp_max_compressed_bytes_u16",
+    "lscomp_compress_u16", "lscomp_compress_u16_async", "lscomp_decompress_u16",
+    "lscomp_decompress_u16_async", "lscomp_stream_status", "lscomp_reserve_u16",
+    "lscomp_release_all", "lscomp_status_string", "lscomp_version",
+)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C sc25-lscomp_amd")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, sz, u64p = ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)
+    for name in ("lscomp_num_blocks_u16", "lscomp_header_bytes_u16", "lscomp_max_compressed_bytes_u16"):
+        getattr(lib, name).restype = sz
+        getattr(lib, name).argtypes = [Dim3]
+    lib.lscomp_compress_u16.restype = ctypes.c_int
+    lib.lscomp_compress_u16.argtypes = [vp, vp, sz, ctypes.POINTER(sz), Dim3, Bins4, ctypes.c_float, vp]
+    lib.lscomp_compress_u16_async.restype = ctypes.c_int
+    lib.lscomp_compress_u16_async.argtypes = [vp, vp, sz, vp, Dim3, Bins4, ctypes.c_float, vp]
+    lib.lscomp_decompress_u16.restype = ctypes.c_int
+    lib.lscomp_decompress_u16.argtypes = [vp, vp, sz, Dim3, Bins4, vp]
+    lib.lscomp_decompress_u16_async.restype = ctypes.c_int
+    lib.lscomp_decompress_u16_async.argtypes = [vp, vp, sz, Dim3, Bins4, vp]
+    lib.lscomp_stream_status.restype = ctypes.c_int
+    lib.lscomp_stream_status.argtypes = [vp]
+    lib.lscomp_reserve_u16.restype = ctypes.c_int
+    lib.lscomp_reserve_u16.argtypes = [Dim3, vp]
+    lib.lscomp_release_all.restype = None
+    lib.lscomp_release_all.argtypes = []
+    lib.lscomp_status_string.restype = ctypes.c_char_p
+    lib.lscomp_status_string.argtypes = [ctypes.c_int]
+    lib.lscomp_version.restype = ctypes.c_int
+    lib.lscomp_version.argtypes = []
+    _ = u64p
+    return lib
+
+
+_lib = _load()
+
+
+def lib():
+    return _lib
+
+
+def _dims(dims) -> Dim3:
+    x, y, z = (int(v) for v in dims)
+    return Dim3(x, y, z)
+
+
+def _bins(bins) -> Bins4:
+    b = tuple(int(v) for v in bins)
+    if len(b) != 4:
+        raise ValueError("bins must have 4 entries")
+    return Bins4(*b)
+
+
+def num_blocks(dims) -> int:
+    return int(_lib.lscomp_num_blocks_u16(_dims(dims)))
+
+
+def header_bytes(dims) -> int:
+    return int(_lib.lscomp_header_bytes_u16(_dims(dims)))
+
+
+def max_compressed_bytes(dims) -> int:
+    return int(_lib.lscomp_max_compressed_bytes_u16(_dims(dims)))
+
+
+def _stream_handle(stream):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _check_volume(t, dims):
+    x, y, z = (int(v) for v in dims)
+    if not t.is_cuda:
+        raise ValueError("volume must be a device tensor")
+    if t.element_size() != 2 or not t.is_contiguous() or t.numel() != x * y * z:
+        raise ValueError("volume must be a contiguous 2-byte tensor of x*y*z elements")
+
+
+def compress(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, out=None, stream=None):
+    """Compress a device volume; returns (stream tensor uint8, stream bytes).
+
+    ``out`` (uint8 device tensor) defaults to a worst-case sized buffer.
+    Raises LscompError on failure (ERR_CAPACITY if ``out`` is too small).
+    """
+    import torch
+    _check_volume(vol, dims)
+    if out is None:
+        out = torch.empty(max_compressed_bytes(dims), dtype=torch.uint8, device=vol.device)
+    n = ctypes.c_size_t(0)
+    st = _lib.lscomp_compress_u16(ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                  out.numel(), ctypes.byref(n), _dims(dims), _bins(bins),
+                                  ctypes.c_float(pooling), _stream_handle(stream))
+    if st != OK:
+        err = LscompError(st, "lscomp_compress_u16")
+        err.required = int(n.value)
+        raise err
+    return out, int(n.value)
+
+
+def compress_async(vol, dims, bins, pooling, out, d_size, stream=None) -> None:
+    """Stream-ordered compress; the stream size lands in ``d_size`` (uint64/int64 device tensor, 1 element)."""
+    _check_volume(vol, dims)
+    st = _lib.lscomp_compress_u16_async(ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                        out.numel(), ctypes.c_void_p(d_size.data_ptr()), _dims(dims),
+                                        _bins(bins), ctypes.c_float(pooling), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_compress_u16_async")
+
+
+def decompress(stream_buf, nbytes: int, dims, bins=LOSSLESS_BINS, out=None, stream=None):
+    """Decompress ``nbytes`` of a device stream into a (x, y, z) int16-typed volume (uint16 bits)."""
+    import torch
+    x, y, z = (int(v) for v in dims)
+    if out is None:
+        out = torch.empty((x, y, z), dtype=torch.int16, device=stream_buf.device)
+    _check_volume(out, dims)
+    st = _lib.lscomp_decompress_u16(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream_buf.data_ptr()),
+                                    int(nbytes), _dims(dims), _bins(bins), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_decompress_u16")
+    return out
+
+
+def decompress_async(stream_buf, nbytes: int, dims, bins, out, stream=None) -> None:
+    _check_volume(out, dims)
+    st = _lib.lscomp_decompress_u16_async(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream_buf.data_ptr()),
+                                          int(nbytes), _dims(dims), _bins(bins), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_decompress_u16_async")
+
+
+def stream_status(stream=None) -> int:
+    return int(_lib.lscomp_stream_status(_stream_handle(stream)))
+
+
+def reserve(dims, stream=None) -> None:
+    st = _lib.lscomp_reserve_u16(_dims(dims), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_reserve_u16")
+
+
+def release_all() -> None:
+    _lib.lscomp_release_all()
+
+
+def version() -> int:
+    return int(_lib.lscomp_version())

@@ -1,0 +1,75 @@
+// Host build of the kernels' per-block code (lscomp_block.h, lscomp_params.h)
+// for the CPU test suite: the same source the gfx950 kernels compile, driven
+// one block at a time so it can be compared with the oracle without a GPU.
+#include <stdint.h>
+#include "lscomp_block.h"
+#include "lscomp_params.h"
+
+using namespace lsc;
+
+static void pack_rows(const uint16_t v[64], uint32_t D[32])
+{
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 4; c++)
+            D[4 * r + c] = (uint32_t)v[8 * r + 2 * c] | ((uint32_t)v[8 * r + 2 * c + 1] << 16);
+}
+
+extern "C" {
+
+// Header byte and payload bytes of one block; payload written to out (<= 128 B).
+uint32_t bh_encode(const uint16_t v[64], const uint32_t bins[4], float p, uint8_t *out, uint32_t *hdr)
+{
+    Decide d = make_decide(bins, p);
+    uint32_t D[32];
+    pack_rows(v, D);
+    uint32_t h = block_header(D, d);
+    uint32_t words[32];
+    uint32_t n = encode_block(D, h, d, words);
+    for (uint32_t i = 0; i < n; i++)
+        for (int b = 0; b < 4; b++) out[4 * i + b] = (uint8_t)(words[i] >> (8 * b));
+    *hdr = h;
+    return 4 * n;
+}
+
+void bh_decode(uint32_t hdr, const uint8_t *payload, const uint32_t bins[4], uint16_t v[64])
+{
+    uint32_t words[32] = {0};
+    uint32_t n = header_payload_bytes(hdr) / 4;
+    for (uint32_t i = 0; i < n && i < 32; i++)
+        words[i] = (uint32_t)payload[4 * i] | ((uint32_t)payload[4 * i + 1] << 8) |
+                   ((uint32_t)payload[4 * i + 2] << 16) | ((uint32_t)payload[4 * i + 3] << 24);
+    uint32_t D[32];
+    if ((hdr & 0x1f) == 0) {
+        for (int i = 0; i < 32; i++) D[i] = 0;
+    } else {
+        decode_block(words, hdr, bins, D);
+    }
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 4; c++) {
+            v[8 * r + 2 * c] = (uint16_t)(D[4 * r + c] & 0xffff);
+            v[8 * r + 2 * c + 1] = (uint16_t)(D[4 * r + c] >> 16);
+        }
+}
+
+uint32_t bh_fdiv(uint32_t n, uint32_t d) { return fdiv(n, make_fastdiv(d)); }
+
+// Counts (v, d) pairs with v in [0, 65536) and d in [d_lo, d_hi] where the
+// kernel's division q = trunc((float)v * f_d) differs from v / d.
+uint64_t bh_quant_mismatches(uint32_t d_lo, uint32_t d_hi)
+{
+    uint64_t bad = 0;
+    for (uint64_t d = d_lo; d <= d_hi; d++) {
+        const uint32_t b4[4] = {(uint32_t)d, (uint32_t)d, (uint32_t)d, (uint32_t)d};
+        const float f = make_decide(b4, 1.0f).f[0];
+        for (uint32_t v = 0; v < 65536; v++) bad += quant(v, f) != v / (uint32_t)d;
+    }
+    return bad;
+}
+
+uint32_t bh_zth(float p)
+{
+    const uint32_t b4[4] = {1, 1, 1, 1};
+    return make_decide(b4, p).zth;
+}
+
+}  // extern "C"

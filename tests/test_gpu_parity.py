@@ -1,0 +1,166 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bit-exact streams and volumes on seeded inputs at sizes the oracle handles in
+seconds, edge shapes (partial blocks on both axes, tiny volumes, odd widths,
+misaligned pointers), multi-tile look-back chains, capacity and corruption
+handling, and scratch reuse across launches and streams.
+"""
+import numpy as np
+import pytest
+
+from lsc_helpers import MODES, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(vol):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(vol).view(np.int16)).cuda()
+
+
+def check_case(gpu, orc, vol, bins, p):
+    dims = vol.shape
+    d = to_dev(vol)
+    out, n = gpu.compress(d, dims, bins, p)
+    st, ref, refn = orc.compress(vol, bins, p)
+    assert st == 0
+    assert n == refn, f"stream size {n} != oracle {refn}"
+    got = out[:n].cpu().numpy()
+    if not np.array_equal(got, ref):
+        bad = np.nonzero(got != ref)[0]
+        pytest.fail(f"stream differs at {bad.size} bytes, first {bad[:8]} (R={orc.header_bytes(dims)})")
+    dec = gpu.decompress(out, n, dims, bins)
+    st, refdec = orc.decompress(ref, dims, bins)
+    assert st == 0
+    assert np.array_equal(dec.cpu().numpy().view(np.uint16), refdec)
+    if tuple(bins) == (1, 1, 1, 1) and p >= 1.0:
+        assert np.array_equal(refdec, vol)
+    return n
+
+
+SHAPES = [(1, 8, 8), (1, 1, 1), (1, 8, 9), (2, 13, 17), (1, 9, 9), (1, 8, 1), (3, 64, 64),
+          (1, 100, 101), (2, 50, 1558), (1, 1024, 1024), (16, 512, 512)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("kind", ["cssi", "uniform", "mixed"])
+@pytest.mark.parametrize("mode", ["lossless", "b3-5-10-15_p0.5"])
+def test_shapes(gpu, orc, shape, kind, mode):
+    bins, p = MODES[mode]
+    seed = (shape[0] * 31 + shape[1] * 7 + shape[2] + len(kind)) % 1000
+    check_case(gpu, orc, synth(kind, shape, seed=seed), bins, p)
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+@pytest.mark.parametrize("kind", ["sparse", "cssi", "xpcs", "mixed", "zeros", "max"])
+def test_modes(gpu, orc, mode, kind):
+    bins, p = MODES[mode]
+    check_case(gpu, orc, synth(kind, (4, 136, 200), seed=7), bins, p)
+
+
+def test_cfg1_sparse_lossless(gpu, orc):
+    """16x512x512 Poisson(0.05) + hot pixels, lossless: exact round trip, bytes == oracle."""
+    vol = synth("sparse", (16, 512, 512), seed=1)
+    check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+
+
+def test_long_lookback_chain(gpu, orc):
+    """1024 tiles in one stream; the look-back must resolve across many probes."""
+    vol = synth("xpcs", (64, 512, 512), seed=3)
+    check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+    check_case(gpu, orc, vol, (5, 5, 5, 5), 0.5)
+
+
+@pytest.mark.parametrize("offset_elems", [1, 2, 8])
+def test_misaligned_input(gpu, orc, offset_elems):
+    """Volumes at 2-, 4- and 16-byte offsets inside a larger buffer use other load paths."""
+    import torch
+    vol = synth("cssi", (2, 64, 96), seed=11)
+    big = torch.zeros(vol.size + 16, dtype=torch.int16, device="cuda")
+    big[offset_elems:offset_elems + vol.size] = to_dev(vol).reshape(-1)
+    d = big[offset_elems:offset_elems + vol.size]
+    for bins, p in (MODES["lossless"], MODES["b3-5-10-15_p0.5"]):
+        out, n = gpu.compress(d, vol.shape, bins, p)
+        st, ref, refn = orc.compress(vol, bins, p)
+        assert n == refn and np.array_equal(out[:n].cpu().numpy(), ref)
+        decbig = torch.full((vol.size + 16,), 7, dtype=torch.int16, device="cuda")
+        dv = decbig[offset_elems:offset_elems + vol.size]
+        gpu.decompress(out, n, vol.shape, bins, out=dv)
+        _, refdec = orc.decompress(ref, vol.shape, bins)
+        host = decbig.cpu().numpy().view(np.uint16)
+        assert np.array_equal(host[offset_elems:offset_elems + vol.size].reshape(vol.shape), refdec)
+        assert np.all(host[:offset_elems] == 7) and np.all(host[offset_elems + vol.size:] == 7)
+
+
+def test_capacity_short(gpu, orc):
+    """A short output buffer: error, required size reported, nothing written past capacity."""
+    import torch
+    vol = synth("uniform", (1, 64, 64), seed=5)
+    st, ref, refn = orc.compress(vol, (1, 1, 1, 1), 1.0)
+    cap = refn - 100
+    buf = torch.full((refn + 64,), 0xAB, dtype=torch.uint8, device="cuda")
+    with pytest.raises(gpu.LscompError) as ei:
+        gpu.compress(to_dev(vol), vol.shape, (1, 1, 1, 1), 1.0, out=buf[:cap])
+    assert ei.value.status == gpu.ERR_CAPACITY
+    assert ei.value.required == refn
+    assert np.all(buf[cap:].cpu().numpy() == 0xAB)
+    # the exact size is enough
+    out, n = gpu.compress(to_dev(vol), vol.shape, (1, 1, 1, 1), 1.0, out=buf[:refn])
+    assert n == refn and np.array_equal(out[:n].cpu().numpy(), ref)
+
+
+def test_corrupt_streams(gpu, orc):
+    import torch
+    vol = synth("cssi", (2, 64, 64), seed=9)
+    dims = vol.shape
+    st, ref, refn = orc.compress(vol, (1, 1, 1, 1), 1.0)
+    d = torch.from_numpy(ref.copy()).cuda()
+    with pytest.raises(gpu.LscompError) as ei:          # truncated payload
+        gpu.decompress(d, refn - 8, dims)
+    assert ei.value.status == gpu.ERR_CORRUPT
+    bad = ref.copy()
+    bad[3] = 0x1f                                        # rate 31 > 16
+    with pytest.raises(gpu.LscompError) as ei:
+        gpu.decompress(torch.from_numpy(bad).cuda(), refn, dims)
+    assert ei.value.status == gpu.ERR_CORRUPT
+    # the sticky error was cleared: a good stream decodes again
+    dec = gpu.decompress(d, refn, dims)
+    assert np.array_equal(dec.cpu().numpy().view(np.uint16), vol)
+
+
+def test_repeated_launches_and_streams(gpu, orc):
+    """Scratch reuse: many launches in a row and on two streams give identical bytes."""
+    import torch
+    vol = synth("xpcs", (8, 256, 256), seed=21)
+    dims = vol.shape
+    st, ref, refn = orc.compress(vol, (3, 5, 10, 15), 0.5)
+    d = to_dev(vol)
+    for _ in range(40):
+        out, n = gpu.compress(d, dims, (3, 5, 10, 15), 0.5)
+        assert n == refn
+    assert np.array_equal(out[:n].cpu().numpy(), ref)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = []
+    for s in (s1, s2):
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            o = torch.empty(gpu.max_compressed_bytes(dims), dtype=torch.uint8, device="cuda")
+            sz = torch.zeros(1, dtype=torch.int64, device="cuda")
+            gpu.compress_async(d, dims, (3, 5, 10, 15), 0.5, o, sz, stream=s)
+            outs.append((o, sz))
+    torch.cuda.synchronize()
+    for o, sz in outs:
+        assert int(sz.item()) == refn
+        assert np.array_equal(o[:refn].cpu().numpy(), ref)
+    assert gpu.stream_status(s1) == gpu.OK and gpu.stream_status(s2) == gpu.OK
+
+
+def test_decode_oracle_stream(gpu, orc):
+    """The GPU decoder reads streams written by the oracle (not only its own)."""
+    import torch
+    vol = synth("mixed", (3, 40, 72), seed=4)
+    for bins, p in MODES.values():
+        st, ref, refn = orc.compress(vol, bins, p)
+        dec = gpu.decompress(torch.from_numpy(ref.copy()).cuda(), refn, vol.shape, bins)
+        _, refdec = orc.decompress(ref, vol.shape, bins)
+        assert np.array_equal(dec.cpu().numpy().view(np.uint16), refdec)
