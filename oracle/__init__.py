@@ -95,13 +95,13 @@ class Oracle:
     def encode_block(self, v64, hdr: int, bins) -> bytes:
         v = np.ascontiguousarray(v64, dtype=np.uint32).reshape(64)
         b = np.ascontiguousarray(bins, dtype=np.uint32)
-        out = np.zeros(128, np.uint8)
+        out = np.zeros(256, np.uint8)
         n = self.lib.oracle_encode_block(v.ctypes.data_as(_u32p), ctypes.c_uint8(hdr),
                                          b.ctypes.data_as(_u32p), out.ctypes.data_as(_u8p))
         return out[:n].tobytes()
 
     def decode_block(self, hdr: int, payload: bytes, bins) -> np.ndarray:
-        pl = np.zeros(128, np.uint8)
+        pl = np.zeros(256, np.uint8)
         pl[: len(payload)] = np.frombuffer(payload, np.uint8)
         b = np.ascontiguousarray(bins, dtype=np.uint32)
         v = np.zeros(64, np.uint32)

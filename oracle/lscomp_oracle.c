@@ -297,7 +297,7 @@ int oracle_compress_u16(const uint16_t *in, uint32_t x, uint32_t y, uint32_t z,
     uint64_t pos = hb;
     for (uint64_t i = nb; i < hb; i++) out[i] = 0;
     uint32_t v[64];
-    uint8_t tmp[128];
+    uint8_t tmp[256];   /* 128 suffices for rate <= 16; slack for the mutants */
     int over = 0;
     for (uint64_t b = 0; b < nb; b++) {
         uint32_t pb;

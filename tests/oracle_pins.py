@@ -1,0 +1,278 @@
+"""Pins for the CPU oracle: checks that fix its behaviour independently of its code.
+
+Each ``pin_*`` takes an ``oracle.Oracle`` and raises AssertionError when the
+oracle disagrees with:
+  * hand-derived known-answer bytes (single 8x8 frames worked out from the
+    reference kernel's arithmetic, lsCOMP/src/cuLSZ_kernel.cu:898-1277);
+  * the bit layout implied by the reference's shifts (one-hot inputs: exactly
+    one payload bit, at plane i, byte r, bit 7-k; pooled: byte t//8, bit 7-t%8);
+  * decision boundaries computed by hand (pooling threshold in float,
+    progressive bin choice, rate = bit width);
+  * invariants (lossless round trip, lossy error bounds, stream size and
+    offsets, header locality, frame-prefix property, serial == OpenMP bytes,
+    block gather == numpy reshape of the zero-padded volume).
+The same pins are run against oracle builds with one planted mistake each;
+every mutant must fail at least one pin.
+"""
+import numpy as np
+
+LOSSLESS = (1, 1, 1, 1)
+
+
+def _stream(orc, vol, bins, p):
+    st, s, n = orc.compress(np.asarray(vol, np.uint16), bins, p)
+    assert st == 0, f"compress status {st}"
+    return bytes(s[:n])
+
+
+# ---- hand-derived known answers ------------------------------------------
+
+def pin_kat_bytes(orc):
+    z = np.zeros((1, 8, 8), np.uint16)
+    # all zeros, lossless: bc=3 (all counts 64), rate 0 -> 0x60, R = 4
+    assert _stream(orc, z, LOSSLESS, 1.0) == bytes([0x60, 0, 0, 0])
+    # v[0]=5, lossless: rate 3, planes 0..2 carry 1,0,1 in byte 0 bit 7
+    v = z.copy(); v[0, 0, 0] = 5
+    plane1 = bytes([0x80] + [0] * 7)
+    assert _stream(orc, v, LOSSLESS, 1.0) == bytes([0x63, 0, 0, 0]) + plane1 + bytes(8) + plane1
+    # v[0]=5, bins 3 5 10 15, p 0.5: zc=63 -> pool; counts (63,63,64,64) -> bc=1;
+    # pair max floor(5/2)=2, 2/5=0 -> rate 0 -> 0x80|1<<5 = 0xA0
+    assert _stream(orc, v, (3, 5, 10, 15), 0.5) == bytes([0xA0, 0, 0, 0])
+    # v[0]=5, bins 1 1 1 1, p 0: pool (zc=63 > 0); qp[0] = 2 -> rate 2, hdr 0xE2;
+    # plane 0 all zero, plane 1 byte 0 bit 7
+    assert _stream(orc, v, LOSSLESS, 0.0) == bytes([0xE2, 0, 0, 0]) + bytes(4) + bytes([0x80, 0, 0, 0])
+    # pair (1, 0) pooled: floor(1/2) = 0 -> rate 0 -> 0xE0; pair (3, 0): floor(3/2) = 1 -> 0xE1
+    v1 = z.copy(); v1[0, 0, 0] = 1
+    assert _stream(orc, v1, LOSSLESS, 0.0) == bytes([0xE0, 0, 0, 0])
+    v3 = z.copy(); v3[0, 0, 0] = 3
+    assert _stream(orc, v3, LOSSLESS, 0.0) == bytes([0xE1, 0, 0, 0, 0x80, 0, 0, 0])
+    # all 7, bins 3 5 10 15, p 0.5: no zeros; counts (0,0,64,64) -> bc=1; 7/5=1 -> 0x21, one FF plane
+    s = np.full((1, 8, 8), 7, np.uint16)
+    assert _stream(orc, s, (3, 5, 10, 15), 0.5) == bytes([0x21, 0, 0, 0]) + b"\xff" * 8
+    # all 65535 lossless: rate 16 -> 0x70, 16 planes of FF
+    m = np.full((1, 8, 8), 65535, np.uint16)
+    assert _stream(orc, m, LOSSLESS, 1.0) == bytes([0x70, 0, 0, 0]) + b"\xff" * 128
+    # 1x8x9: two blocks, R = 4
+    assert _stream(orc, np.zeros((1, 8, 9), np.uint16), LOSSLESS, 1.0) == bytes([0x60, 0x60, 0, 0])
+    # 1x8x1: one block, the 7 missing columns are zero padding
+    c = np.arange(1, 9, dtype=np.uint16).reshape(1, 8, 1)   # rows 0..7 hold 1..8 in column 0
+    s = _stream(orc, c, LOSSLESS, 1.0)
+    assert s[0] == 0x64 and len(s) == 4 + 32
+    for i in range(4):   # plane i byte r bit 7 = bit i of (r+1)
+        assert s[4 + 8 * i: 12 + 8 * i] == bytes([0x80 if ((r + 1) >> i) & 1 else 0 for r in range(8)])
+
+
+def pin_kat_decode(orc):
+    # the 0x21 block decodes to 1*5 everywhere; the 0xA0 block to zeros;
+    # pooled 0xE2 block decodes both elements of pair 0 to 2
+    assert np.all(orc.decode_block(0x21, b"\xff" * 8, (3, 5, 10, 15)) == 5)
+    assert np.all(orc.decode_block(0xA0, b"", (3, 5, 10, 15)) == 0)
+    d = orc.decode_block(0xE2, bytes(4) + bytes([0x80, 0, 0, 0]), LOSSLESS)
+    assert d[0] == 2 and d[1] == 2 and np.all(d[2:] == 0)
+
+
+# ---- bit layout -----------------------------------------------------------
+
+def pin_onehot_plain(orc):
+    for i in range(16):
+        for r in range(8):
+            for k in range(8):
+                v = np.zeros(64, np.uint32)
+                v[8 * r + k] = 1 << i
+                h, nb = orc.block_header(v, LOSSLESS, 1.0)
+                assert h == 0x60 | (i + 1) and nb == 8 * (i + 1), (i, r, k, h)
+                pl = orc.encode_block(v, h, LOSSLESS)
+                exp = bytearray(8 * (i + 1))
+                exp[8 * i + r] = 0x80 >> k
+                assert pl == bytes(exp), (i, r, k)
+                assert np.array_equal(orc.decode_block(h, pl, LOSSLESS), v)
+
+
+def pin_onehot_pooled(orc):
+    for i in range(16):
+        for t in range(32):
+            r, c = divmod(t, 4)
+            v = np.zeros(64, np.uint32)
+            v[8 * r + 2 * c] = v[8 * r + 2 * c + 1] = 1 << i
+            h, nb = orc.block_header(v, LOSSLESS, 0.0)   # zc = 62 > 0 -> pooled
+            assert h == 0xE0 | (i + 1) and nb == 4 * (i + 1), (i, t, h)
+            pl = orc.encode_block(v, h, LOSSLESS)
+            exp = bytearray(4 * (i + 1))
+            exp[4 * i + t // 8] = 0x80 >> (t % 8)
+            assert pl == bytes(exp), (i, t)
+            assert np.array_equal(orc.decode_block(h, pl, LOSSLESS), v)
+
+
+# ---- decisions ------------------------------------------------------------
+
+# smallest zero count that pools: (float)zc / 64 > p evaluated by hand
+POOL_THRESHOLDS = {0.0: 1, 0.25: 17, 0.3: 20, 0.5: 33, 0.7: 45, 1.0: None}
+
+
+def pin_pool_threshold(orc):
+    for p, zmin in POOL_THRESHOLDS.items():
+        for zc in range(0, 65):
+            v = np.zeros(64, np.uint32)
+            v[zc:] = 3
+            h, _ = orc.block_header(v, LOSSLESS, p)
+            want = zmin is not None and zc >= zmin
+            assert bool(h & 0x80) == want, (p, zc, hex(h))
+
+
+def pin_bin_choice(orc):
+    bins = (3, 5, 10, 15)
+    cases = [  # (nonzero values, expected bin choice)
+        ([], 3), ([1, 2], 3), ([3], 0), ([4], 0), ([5], 1), ([9], 1), ([10], 2), ([14], 2),
+        ([15], 3), ([60000], 3), ([3, 20], 0), ([7, 20], 1), ([12, 20], 2), ([2, 12], 2),
+        ([4, 7, 12], 0), ([7, 12], 1),
+    ]
+    for vals, bc in cases:
+        v = np.zeros(64, np.uint32)
+        v[: len(vals)] = vals
+        h, _ = orc.block_header(v, bins, 1.0)
+        assert (h >> 5) & 3 == bc, (vals, hex(h))
+        assert h & 0x1f == (max(vals, default=0) // bins[bc]).bit_length(), (vals, hex(h))
+    # equal bins always choose 3; huge bins quantize everything to 0
+    v = np.arange(64, dtype=np.uint32) * 1000
+    assert (orc.block_header(v, (7, 7, 7, 7), 1.0)[0] >> 5) & 3 == 3
+    assert orc.block_header(v, (70000, 80000, 90000, 100000), 1.0)[0] == 0x60
+
+
+def pin_rate_edges(orc):
+    for i in range(17):
+        for val in {max(0, (1 << i) - 1), min(65535, 1 << i)}:
+            v = np.zeros(64, np.uint32)
+            v[17] = val
+            h, nb = orc.block_header(v, LOSSLESS, 1.0)
+            assert h & 0x1f == int(val).bit_length() and nb == 8 * int(val).bit_length(), (val, hex(h))
+
+
+# ---- invariants -----------------------------------------------------------
+
+def _volumes():
+    rng = np.random.default_rng(1234)
+    vols = []
+    for shape in [(1, 8, 8), (2, 13, 17), (1, 9, 9), (3, 24, 40), (1, 8, 1), (2, 31, 66)]:
+        vols.append(rng.poisson(0.4, size=shape).astype(np.uint16))
+        u = rng.integers(0, 65536, size=shape).astype(np.uint16)
+        u[rng.random(shape) < 0.5] = 0
+        vols.append(u)
+        vols.append((rng.integers(0, 2, size=shape) * rng.integers(0, 40, size=shape)).astype(np.uint16))
+    return vols
+
+
+def _blocks_numpy(vol):
+    """(B, 64) blocks of a volume by zero padding and reshaping (independent of the oracle's indexing)."""
+    x, y, z = vol.shape
+    py, pz = -(-y // 8) * 8, -(-z // 8) * 8
+    pad = np.zeros((x, py, pz), np.uint32)
+    pad[:, :y, :z] = vol
+    return pad.reshape(x, py // 8, 8, pz // 8, 8).transpose(0, 1, 3, 2, 4).reshape(-1, 64)
+
+
+def _sizes(hdrs):
+    hdrs = np.asarray(hdrs, np.int64)
+    return (hdrs & 0x1f) * np.where(hdrs & 0x80, 4, 8)
+
+
+def pin_lossless_roundtrip(orc):
+    for vol in _volumes():
+        st, s, n = orc.compress(vol, LOSSLESS, 1.0)
+        st2, dec = orc.decompress(s[:n], vol.shape, LOSSLESS, fill=0xBEEF)
+        assert st == 0 and st2 == 0 and np.array_equal(dec, vol)
+
+
+def pin_stream_structure(orc):
+    for vol in _volumes():
+        for bins, p in ((LOSSLESS, 1.0), ((3, 5, 10, 15), 0.5), ((2, 2, 2, 2), 0.0)):
+            st, s, n = orc.compress(vol, bins, p)
+            blocks = _blocks_numpy(vol)
+            nb = len(blocks)
+            R = -(-nb // 4) * 4
+            assert orc.num_blocks(vol.shape) == nb and orc.header_bytes(vol.shape) == R
+            assert orc.max_compressed_bytes(vol.shape) == R + 128 * nb
+            hdrs = np.frombuffer(bytes(s[:nb]), np.uint8)
+            assert all(b == 0 for b in bytes(s[nb:R]))          # pad bytes
+            sizes = _sizes(hdrs)
+            assert n == R + int(sizes.sum())                  # size formula
+            offs = R + np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            for b in range(nb):                               # headers from numpy-gathered blocks,
+                h, nbytes = orc.block_header(blocks[b], bins, p)   # payload at the scanned offset
+                assert h == hdrs[b] and nbytes == sizes[b]
+                assert bytes(s[offs[b]: offs[b] + sizes[b]]) == orc.encode_block(blocks[b], h, bins)
+
+
+def pin_lossy_bounds(orc):
+    for vol in _volumes():
+        for bins, p in (((3, 5, 10, 15), 0.5), ((3, 4, 5, 6), 0.7), ((2, 9, 9, 30), 0.0), ((5, 5, 5, 5), 1.0)):
+            st, s, n = orc.compress(vol, bins, p)
+            st2, dec = orc.decompress(s[:n], vol.shape, bins)
+            blocks, dblocks = _blocks_numpy(vol).astype(np.int64), _blocks_numpy(dec).astype(np.int64)
+            inside = _blocks_numpy(np.ones(vol.shape, np.uint16)).astype(bool)   # not padding
+            hdrs = np.frombuffer(bytes(s[: len(blocks)]), np.uint8)
+            for b in range(len(blocks)):
+                h, v, w, m = int(hdrs[b]), blocks[b], dblocks[b], inside[b]
+                bin_ = bins[(h >> 5) & 3]
+                nz = int((v == 0).sum())                     # padding counts as zeros
+                assert bool(h & 0x80) == (np.float32(nz) / np.float32(64) > np.float32(p))
+                if h & 0x80:
+                    a, c = v[0::2], v[1::2]
+                    both = m[0::2] & m[1::2]
+                    assert np.array_equal(w[0::2][both], w[1::2][both])
+                    bound = (np.abs(a - c) + 1) // 2 + bin_ - 1
+                    assert np.all((np.abs(a - w[0::2]) <= bound)[m[0::2]])
+                    assert np.all((np.abs(c - w[1::2]) <= bound)[m[1::2]])
+                else:
+                    d = (v - w)[m]
+                    assert np.all(d >= 0) and np.all(d <= bin_ - 1), (b, hex(h))
+
+
+def pin_header_locality(orc):
+    rng = np.random.default_rng(5)
+    vol = rng.poisson(0.5, size=(2, 24, 24)).astype(np.uint16)
+    st, s, n = orc.compress(vol, (3, 5, 10, 15), 0.5)
+    vol2 = vol.copy()
+    vol2[1, 8:16, 8:16] = rng.integers(0, 65536, size=(8, 8))    # block 4 of frame 1 only
+    st, s2, n2 = orc.compress(vol2, (3, 5, 10, 15), 0.5)
+    nb = orc.num_blocks(vol.shape)
+    j = 9 + 4
+    h1, h2 = np.frombuffer(bytes(s[:nb]), np.uint8), np.frombuffer(bytes(s2[:nb]), np.uint8)
+    assert np.array_equal(np.delete(h1, j), np.delete(h2, j))
+    R = orc.header_bytes(vol.shape)
+    pre = R + int(_sizes(h1[:j]).sum())
+    assert bytes(s[R:pre]) == bytes(s2[R:pre])
+    tail1 = bytes(s[pre + int(_sizes(h1[j:j + 1]).sum()):n])
+    tail2 = bytes(s2[pre + int(_sizes(h2[j:j + 1]).sum()):n2])
+    assert tail1 == tail2
+
+
+def pin_frame_prefix(orc):
+    rng = np.random.default_rng(6)
+    vol = rng.poisson(0.3, size=(5, 20, 28)).astype(np.uint16)
+    st, s, n = orc.compress(vol, (3, 5, 10, 15), 0.5)
+    nb, R = orc.num_blocks(vol.shape), orc.header_bytes(vol.shape)
+    for k in (1, 2, 4):
+        st, sk, nk = orc.compress(vol[:k], (3, 5, 10, 15), 0.5)
+        nbk, Rk = orc.num_blocks(vol[:k].shape), orc.header_bytes(vol[:k].shape)
+        assert bytes(sk[:nbk]) == bytes(s[:nbk])
+        assert bytes(sk[Rk:nk]) == bytes(s[R: R + nk - Rk])
+
+
+def pin_omp_equals_serial(orc):
+    for vol in _volumes()[::3]:
+        for bins, p in ((LOSSLESS, 1.0), ((3, 5, 10, 15), 0.5)):
+            a = orc.compress(vol, bins, p)
+            b = orc.compress(vol, bins, p, omp=True)
+            assert a[0] == b[0] == 0 and a[2] == b[2] and np.array_equal(a[1], b[1])
+            assert np.array_equal(orc.decompress(a[1], vol.shape, bins)[1],
+                                  orc.decompress(a[1], vol.shape, bins, omp=True)[1])
+
+
+ALL_PINS = [pin_kat_bytes, pin_kat_decode, pin_onehot_plain, pin_onehot_pooled, pin_pool_threshold,
+            pin_bin_choice, pin_rate_edges, pin_lossless_roundtrip, pin_stream_structure, pin_lossy_bounds,
+            pin_header_locality, pin_frame_prefix, pin_omp_equals_serial]
+
+# planted mistakes in oracle/lscomp_oracle.c (ORACLE_MUTANT=n)
+MUTANTS = {1: "plain bits LSB-first", 2: "plain block transposed", 3: "pool on >=", 4: "bin counts use <=",
+           5: "no 4-byte header pad", 6: "pair average rounds up", 7: "rate one too high",
+           8: "pooled decode drops the second pair element"}

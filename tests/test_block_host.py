@@ -1,0 +1,100 @@
+"""The kernels' per-block code (lscomp_block.h) built for the host, against the oracle.
+
+The gfx950 kernels and this host build compile the same header; only
+v_perm_b32 / v_alignbit_b32 are emulated on the host.  The GPU parity tests
+cover the device build of the same functions.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from lsc_helpers import MODES
+
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u16p = ctypes.POINTER(ctypes.c_uint16)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def _blocks(rng, n):
+    out = []
+    for i in range(n):
+        kind = i % 7
+        if kind == 0:
+            v = rng.poisson(0.3, 64)
+        elif kind == 1:
+            v = rng.integers(0, 65536, 64)
+        elif kind == 2:
+            v = rng.integers(0, 16, 64) * (rng.random(64) < 0.3)
+        elif kind == 3:
+            v = np.full(64, rng.integers(0, 65536))
+        elif kind == 4:
+            v = rng.integers(0, 2 ** int(rng.integers(1, 17)), 64)
+        elif kind == 5:
+            v = np.zeros(64)
+            v[rng.integers(0, 64)] = rng.integers(0, 65536)
+        else:
+            v = rng.poisson(0.05, 64) * rng.integers(1, 4000)
+        out.append(np.clip(v, 0, 65535).astype(np.uint16))
+    return out
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_encode_decode_vs_oracle(block_host, orc, mode):
+    bins, p = MODES[mode]
+    b = np.array(bins, np.uint32)
+    rng = np.random.default_rng(sum(map(ord, mode)))
+    for v in _blocks(rng, 3000):
+        h, _ = orc.block_header(v.astype(np.uint32), b, p)
+        ref = orc.encode_block(v.astype(np.uint32), h, b)
+        out = np.zeros(128, np.uint8)
+        hh = ctypes.c_uint32(0)
+        n = block_host.bh_encode(v.ctypes.data_as(u16p), b.ctypes.data_as(u32p), ctypes.c_float(p),
+                                 out.ctypes.data_as(u8p), ctypes.byref(hh))
+        assert hh.value == h, (v, hex(h), hex(hh.value))
+        assert out[:n].tobytes() == ref
+        pl = np.zeros(128, np.uint8)
+        pl[: len(ref)] = np.frombuffer(ref, np.uint8)
+        dec = np.zeros(64, np.uint16)
+        block_host.bh_decode(h, pl.ctypes.data_as(u8p), b.ctypes.data_as(u32p), dec.ctypes.data_as(u16p))
+        assert np.array_equal(dec, orc.decode_block(h, ref, b).astype(np.uint16))
+
+
+def test_decode_random_payloads(block_host, orc):
+    """Arbitrary (also non-canonical) headers and payloads decode like the oracle, incl. mod-2^16 products."""
+    rng = np.random.default_rng(99)
+    for bins in [(1, 1, 1, 1), (3, 5, 10, 15), (7, 300, 5000, 70000)]:
+        b = np.array(bins, np.uint32)
+        for _ in range(2000):
+            h = int(rng.integers(0, 256)) & 0xF0 | int(rng.integers(0, 17))
+            h = (h & 0xE0) | min(h & 0x1F, 16)
+            pl = rng.integers(0, 256, 128).astype(np.uint8)
+            dec = np.zeros(64, np.uint16)
+            block_host.bh_decode(h, pl.ctypes.data_as(u8p), b.ctypes.data_as(u32p), dec.ctypes.data_as(u16p))
+            ref = orc.decode_block(h, pl.tobytes(), b).astype(np.uint64) & 0xFFFF
+            assert np.array_equal(dec, ref.astype(np.uint16)), hex(h)
+
+
+def test_pool_threshold_parameter(block_host):
+    from oracle_pins import POOL_THRESHOLDS
+    for p, zmin in POOL_THRESHOLDS.items():
+        assert block_host.bh_zth(p) == (65 if zmin is None else zmin)
+
+
+def test_fastdiv(block_host):
+    rng = np.random.default_rng(0)
+    ds = list(range(1, 2000)) + [int(x) for x in rng.integers(1, 2 ** 32, 2000)]
+    ds += [2 ** 31, 2 ** 31 + 1, 2 ** 32 - 1, 2 ** 32 - 2, 65536, 65537, 195, 227 * 195]
+    for d in ds:
+        ns = [0, 1, d - 1, d, d + 1, 2 ** 32 - 1, 2 ** 32 - 2, 2 ** 31, 2 ** 31 - 1]
+        ns += [int(x) for x in rng.integers(0, 2 ** 32, 16)]
+        for n in ns:
+            n %= 2 ** 32
+            assert block_host.bh_fdiv(n, d) == n // d, (n, d)
+
+
+def test_quantize_exhaustive(block_host):
+    """q = trunc((float)v * f_d) == v // d for every v < 2^16 and every d <= 65536, and large d."""
+    assert block_host.bh_quant_mismatches(1, 65536) == 0
+    assert block_host.bh_quant_mismatches(65537, 66000) == 0
+    assert block_host.bh_quant_mismatches(4294967000, 4294967295) == 0
