@@ -98,6 +98,56 @@ int lscomp_reserve_u16(lscomp_dim3 dims, hipStream_t stream);
 /* Releases every scratch area of the current device (synchronizes). */
 void lscomp_release_all(void);
 
+/* Launch shape of later calls (process-wide): direction 0 = compression,
+ * 1 = decompression; tile = 8x8 blocks per workgroup (128, 256 or 512;
+ * 0 = default 256); wg_per_cu = workgroups per CU (0 = occupancy limit).
+ * Streams are identical for every shape; only speed changes.  No counterpart
+ * in the reference (its grid is fixed, cuLSZ_entry.cu:100-101); set by the
+ * autotuner or by hand. */
+int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu);
+int lscomp_get_launch_shape(int direction, uint32_t *tile, uint32_t *wg_per_cu);
+
+/* ---- Batched form -------------------------------------------------------
+ * Many independent volumes (e.g. detector frames of uneven sizes), each with
+ * its own stream in exactly the plain format, processed by ONE persistent
+ * kernel launch that pulls tiles of every item from a shared queue.  No
+ * counterpart in the reference (one volume per launch there); a plan's item
+ * streams are byte-identical to lscomp_compress_u16 on each item alone.
+ *
+ * A plan owns its device tables and look-back scratch (allocated once by
+ * lscomp_plan_create); lscomp_plan_set_items_u16 refills it without
+ * allocating, so one plan serves batch after batch.  Use one stream per plan. */
+typedef struct lscomp_item_u16 {
+    lscomp_dim3 dims;          /* x frames of y rows of z pixels */
+    lscomp_bins4 bins;
+    float pooling_th;          /* compression only */
+    uint16_t *d_volume;        /* compress: input volume; decompress: output volume */
+    uint8_t *d_stream;         /* compress: output stream; decompress: input stream */
+    uint64_t stream_bytes;     /* compress: capacity of d_stream; decompress: stream length */
+} lscomp_item_u16;
+
+typedef struct lscomp_plan_s *lscomp_plan;
+
+/* direction 0 = compress, 1 = decompress.  Capacity: up to max_items items and
+ * max_blocks 8x8 blocks in total per batch.  Uses the launch shape of the
+ * direction at creation time (lscomp_set_launch_shape). */
+int lscomp_plan_create(int direction, uint32_t max_items, uint64_t max_blocks, lscomp_plan *plan);
+/* Validates the items and uploads the batch tables (stream-ordered on `stream`;
+ * the host array may be reused when the call returns). */
+int lscomp_plan_set_items_u16(lscomp_plan plan, const lscomp_item_u16 *items, uint32_t n, hipStream_t stream);
+/* One persistent launch over every item of the current batch (stream-ordered). */
+int lscomp_plan_launch(lscomp_plan plan, hipStream_t stream);
+/* The same launch replayed from a hipGraph captured on first use (batches set
+ * later are picked up: the graph reads the tables through pointers). */
+int lscomp_plan_launch_graph(lscomp_plan plan, hipStream_t stream);
+/* Device array of per-item stream sizes (compress) written by the launch. */
+const uint64_t *lscomp_plan_device_sizes(lscomp_plan plan);
+/* Synchronizes `stream` and copies the per-item stream sizes to the host. */
+int lscomp_plan_sizes(lscomp_plan plan, uint64_t *host_sizes, hipStream_t stream);
+/* Synchronizes `stream`; returns and clears errors of the plan's launches. */
+int lscomp_plan_status(lscomp_plan plan, hipStream_t stream);
+int lscomp_plan_destroy(lscomp_plan plan);
+
 const char *lscomp_status_string(int status);
 
 /* Library version, major*10000 + minor*100 + patch. */

@@ -10,6 +10,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -17,8 +18,97 @@
 #include "../../include/lscomp.h"
 #include "lscomp_internal.h"
 #include "lscomp_params.h"
+#include "lscomp_host.h"
 
 using namespace lsc;
+
+namespace lsc {
+// ---------------------------------------------------------------------------
+// Parameters.
+// ---------------------------------------------------------------------------
+bool bins_ok(lscomp_bins4 b)
+{
+    if (b.x == 0 || b.y == 0 || b.z == 0 || b.w == 0) return false;
+    return b.x <= b.y && b.y <= b.z && b.z <= b.w;
+}
+
+uint32_t align_class(const void *vol, uint32_t z)
+{
+    uintptr_t a = (uintptr_t)vol;
+    if (a % 16 == 0 && z % 8 == 0) return ALIGN16;
+    if (a % 4 == 0 && z % 2 == 0) return ALIGN4;
+    return ALIGN2;
+}
+
+uint64_t num_blocks(lscomp_dim3 d)
+{
+    return (uint64_t)d.x * (((uint64_t)d.y + 7) / 8) * (((uint64_t)d.z + 7) / 8);
+}
+
+// Launch shape per direction (0 compress, 1 decompress): tile and workgroups
+// per CU (0 = occupancy limit).  Set by lscomp_set_launch_shape / the autotuner.
+std::atomic<uint32_t> g_tile[2] = {{(uint32_t)kDefaultTile}, {(uint32_t)kDefaultTile}};
+std::atomic<uint32_t> g_wgpcu[2] = {{0u}, {0u}};
+
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return LSCOMP_OK;
+    std::fprintf(stderr, "lscomp: %s failed: %s\n", what, hipGetErrorString(e));
+    return LSCOMP_ERR_HIP;
+}
+
+// Fills the geometry part of an Item; returns a status.
+int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it, uint32_t tile)
+{
+    if (dims.x == 0 || dims.y == 0 || dims.z == 0) return LSCOMP_ERR_INVALID_ARG;
+    if (!bins_ok(bins)) return LSCOMP_ERR_INVALID_ARG;
+    if (!(p >= 0.0f && p <= 1.0f)) return LSCOMP_ERR_INVALID_ARG;
+    const uint64_t nb = num_blocks(dims);
+    if (nb > kMaxBlocksPerItem) return LSCOMP_ERR_TOO_LARGE;
+    std::memset(&it, 0, sizeof it);
+    it.x = dims.x;
+    it.y = dims.y;
+    it.z = dims.z;
+    it.nbz = (dims.z + 7) / 8;
+    it.per_frame = ((dims.y + 7) / 8) * it.nbz;
+    it.nblocks = (uint32_t)nb;
+    it.hdr_bytes = (uint32_t)((nb + 3) / 4 * 4);
+    it.first_tile = 0;
+    it.ntiles = (uint32_t)((nb + tile - 1) / tile);
+    it.div_frame = make_fastdiv(it.per_frame);
+    it.div_nbz = make_fastdiv(it.nbz);
+    const uint32_t b4[4] = {bins.x, bins.y, bins.z, bins.w};
+    it.dec = make_decide(b4, p);
+    return LSCOMP_OK;
+}
+
+LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, uint32_t tile, uint32_t wg_per_cu)
+{
+    static std::atomic<int> occ[2][2][3];   // [compress][table][tile 128/256/512]
+    static std::atomic<int> ncu{0};
+    const int ti = tile == 128 ? 0 : tile == 256 ? 1 : 2;
+    int o = occ[compress][table][ti].load();
+    if (o == 0) {
+        o = workgroups_per_cu(compress, table, (int)tile);
+        occ[compress][table][ti].store(o);
+    }
+    int n = ncu.load();
+    if (n == 0) {
+        n = num_cus();
+        ncu.store(n);
+    }
+    uint32_t per_cu = (uint32_t)o;
+    if (wg_per_cu && wg_per_cu < per_cu) per_cu = wg_per_cu;
+    LaunchCfg c{};
+    c.total_tiles = (uint32_t)total_tiles;
+    c.tile = tile;
+    uint64_t g = (uint64_t)(n > 0 ? n : 1) * per_cu;
+    if (g > total_tiles) g = total_tiles;
+    c.grid = (uint32_t)(g ? g : 1);
+    return c;
+}
+
+}  // namespace lsc
 
 namespace {
 
@@ -41,13 +131,6 @@ std::map<std::pair<int, hipStream_t>, Scratch *> g_scratch;
 
 // Any period below 2^26 launches keeps stale words of an equal epoch out.
 constexpr uint64_t kEpochClearPeriod = 1ull << 25;
-
-int hip_fail(hipError_t e, const char *what)
-{
-    if (e == hipSuccess) return LSCOMP_OK;
-    std::fprintf(stderr, "lscomp: %s failed: %s\n", what, hipGetErrorString(e));
-    return LSCOMP_ERR_HIP;
-}
 
 Scratch *get_scratch(hipStream_t stream)
 {
@@ -109,66 +192,6 @@ int count_launch(Scratch *s)
     return LSCOMP_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Parameters.
-// ---------------------------------------------------------------------------
-bool bins_ok(lscomp_bins4 b)
-{
-    if (b.x == 0 || b.y == 0 || b.z == 0 || b.w == 0) return false;
-    return b.x <= b.y && b.y <= b.z && b.z <= b.w;
-}
-
-uint32_t align_class(const void *vol, uint32_t z)
-{
-    uintptr_t a = (uintptr_t)vol;
-    if (a % 16 == 0 && z % 8 == 0) return ALIGN16;
-    if (a % 4 == 0 && z % 2 == 0) return ALIGN4;
-    return ALIGN2;
-}
-
-uint64_t num_blocks(lscomp_dim3 d)
-{
-    return (uint64_t)d.x * (((uint64_t)d.y + 7) / 8) * (((uint64_t)d.z + 7) / 8);
-}
-
-// Fills the geometry part of an Item; returns a status.
-int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it)
-{
-    if (dims.x == 0 || dims.y == 0 || dims.z == 0) return LSCOMP_ERR_INVALID_ARG;
-    if (!bins_ok(bins)) return LSCOMP_ERR_INVALID_ARG;
-    if (!(p >= 0.0f && p <= 1.0f)) return LSCOMP_ERR_INVALID_ARG;
-    const uint64_t nb = num_blocks(dims);
-    if (nb > kMaxBlocksPerItem) return LSCOMP_ERR_TOO_LARGE;
-    std::memset(&it, 0, sizeof it);
-    it.x = dims.x;
-    it.y = dims.y;
-    it.z = dims.z;
-    it.nbz = (dims.z + 7) / 8;
-    it.per_frame = ((dims.y + 7) / 8) * it.nbz;
-    it.nblocks = (uint32_t)nb;
-    it.hdr_bytes = (uint32_t)((nb + 3) / 4 * 4);
-    it.first_tile = 0;
-    it.ntiles = (uint32_t)((nb + kTile - 1) / kTile);
-    it.div_frame = make_fastdiv(it.per_frame);
-    it.div_nbz = make_fastdiv(it.nbz);
-    const uint32_t b4[4] = {bins.x, bins.y, bins.z, bins.w};
-    it.dec = make_decide(b4, p);
-    return LSCOMP_OK;
-}
-
-LaunchCfg make_cfg(uint64_t total_tiles, bool compress)
-{
-    static int resident[2] = {0, 0};
-    int &r = resident[compress ? 1 : 0];
-    if (r == 0) r = max_resident_workgroups(compress);
-    LaunchCfg c;
-    c.total_tiles = (uint32_t)total_tiles;
-    uint64_t g = total_tiles;
-    if (r > 0 && g > (uint64_t)r) g = (uint64_t)r;
-    c.grid = (uint32_t)(g ? g : 1);
-    return c;
-}
-
 int read_ctrl(Scratch *s)
 {
     int st = hip_fail(hipMemcpyAsync(s->host, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream),
@@ -194,7 +217,7 @@ int take_errors(Scratch *s)
 int prepare_compress(const uint16_t *d_in, uint8_t *d_out, size_t cap, lscomp_dim3 dims, lscomp_bins4 bins,
                      float p, Item &it)
 {
-    int st = make_item(dims, bins, p, it);
+    int st = make_item(dims, bins, p, it, g_tile[0].load());
     if (st) return st;
     if (!d_in || !d_out || ((uintptr_t)d_in & 1) || ((uintptr_t)d_out & 3)) return LSCOMP_ERR_INVALID_ARG;
     if (cap < it.hdr_bytes) return LSCOMP_ERR_CAPACITY;
@@ -208,7 +231,7 @@ int prepare_compress(const uint16_t *d_in, uint8_t *d_out, size_t cap, lscomp_di
 int prepare_decompress(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
                        lscomp_bins4 bins, Item &it)
 {
-    int st = make_item(dims, bins, 0.0f, it);
+    int st = make_item(dims, bins, 0.0f, it, g_tile[1].load());
     if (st) return st;
     if (!d_in || !d_out || ((uintptr_t)d_out & 1) || ((uintptr_t)d_in & 3)) return LSCOMP_ERR_INVALID_ARG;
     if (in_bytes < it.hdr_bytes) return LSCOMP_ERR_CORRUPT;
@@ -219,20 +242,20 @@ int prepare_decompress(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, ls
     return LSCOMP_OK;
 }
 
-// Caller holds s->mu.
+// Caller holds s->mu.  it.ntiles was computed for the direction's tile size.
 int launch_item(Scratch *s, Item &it, bool compress)
 {
+    const int d = compress ? 0 : 1;
     int st = ensure_tiles(s, it.ntiles);
     if (st) return st;
     st = count_launch(s);
     if (st) return st;
+    const LaunchCfg cfg = make_cfg(it.ntiles, compress, false, g_tile[d].load(), g_wgpcu[d].load());
     if (compress) {
         if (!it.result) it.result = &s->ctrl->result;
-        return hip_fail(launch_compress(it, nullptr, 1, make_cfg(it.ntiles, true), s->ctrl, s->status, s->stream),
-                        "compress launch");
+        return hip_fail(launch_compress(it, nullptr, nullptr, cfg, s->ctrl, s->status, s->stream), "compress launch");
     }
-    return hip_fail(launch_decompress(it, nullptr, 1, make_cfg(it.ntiles, false), s->ctrl, s->status, s->stream),
-                    "decompress launch");
+    return hip_fail(launch_decompress(it, nullptr, nullptr, cfg, s->ctrl, s->status, s->stream), "decompress launch");
 }
 
 }  // namespace
@@ -325,7 +348,8 @@ int lscomp_stream_status(hipStream_t stream)
 int lscomp_reserve_u16(lscomp_dim3 dims, hipStream_t stream)
 {
     Item it;
-    int st = make_item(dims, lscomp_bins4{1, 1, 1, 1}, 1.0f, it);
+    const uint32_t tmin = g_tile[0].load() < g_tile[1].load() ? g_tile[0].load() : g_tile[1].load();
+    int st = make_item(dims, lscomp_bins4{1, 1, 1, 1}, 1.0f, it, tmin);
     if (st) return st;
     Scratch *s = get_scratch(stream);
     if (!s) return LSCOMP_ERR_HIP;
@@ -348,6 +372,24 @@ void lscomp_release_all(void)
         delete s;
         itr = g_scratch.erase(itr);
     }
+}
+
+int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu)
+{
+    if (direction != 0 && direction != 1) return LSCOMP_ERR_INVALID_ARG;
+    if (tile == 0) tile = kDefaultTile;
+    if (tile != 128 && tile != 256 && tile != 512) return LSCOMP_ERR_INVALID_ARG;
+    g_tile[direction].store(tile);
+    g_wgpcu[direction].store(wg_per_cu);
+    return LSCOMP_OK;
+}
+
+int lscomp_get_launch_shape(int direction, uint32_t *tile, uint32_t *wg_per_cu)
+{
+    if ((direction != 0 && direction != 1) || !tile || !wg_per_cu) return LSCOMP_ERR_INVALID_ARG;
+    *tile = g_tile[direction].load();
+    *wg_per_cu = g_wgpcu[direction].load();
+    return LSCOMP_OK;
 }
 
 const char *lscomp_status_string(int status)

@@ -6,8 +6,9 @@
 
 namespace lsc {
 
-constexpr int kTile = 256;               // blocks per tile == threads per workgroup
-constexpr int kWaves = kTile / 64;
+// Blocks per tile == threads per workgroup: 128, 256 or 512 (a launch-time
+// choice; the stream does not depend on it).
+constexpr int kDefaultTile = 256;
 constexpr uint32_t kMaxBlocksPerItem = 0x7fffffffu;   // prefix values fit 36-bit words
 
 // Alignment class of a volume: how a row of one block (8 x u16) may be loaded.
@@ -47,7 +48,7 @@ struct Item {
     uint32_t nblocks;         // x * per_frame
     uint32_t hdr_bytes;       // R = 4 * ceil(nblocks/4)
     uint32_t first_tile;      // first global tile of this item
-    uint32_t ntiles;          // ceil(nblocks / kTile)
+    uint32_t ntiles;          // ceil(nblocks / tile)
     uint32_t align;           // ALIGN*
     uint32_t pad0, pad1;
     FastDiv div_frame, div_nbz;
@@ -55,17 +56,21 @@ struct Item {
 };
 
 struct LaunchCfg {
-    uint32_t grid;            // workgroups
+    uint32_t grid;            // workgroups (persistent: each loops over tiles)
     uint32_t total_tiles;
+    uint32_t tile;            // blocks per tile: 128, 256 or 512
+    const uint32_t *d_total_tiles;   // if set, the tile count is read from device memory
 };
 
-// Kernel launchers (lscomp_kernels.hip).  `items` is a device array of
-// nitems descriptors sorted by first_tile, or null for the single item.
-hipError_t launch_compress(const Item &single, const Item *items, uint32_t nitems,
+// Kernel launchers (lscomp_kernels.hip).  `items` is a device array of item
+// descriptors and `tile_items` (device, total_tiles entries) the item of each
+// global tile; both null for the single item passed by value.
+hipError_t launch_compress(const Item &single, const Item *items, const uint32_t *tile_items,
                            LaunchCfg cfg, Ctrl *ctrl, uint64_t *status, hipStream_t stream);
-hipError_t launch_decompress(const Item &single, const Item *items, uint32_t nitems,
+hipError_t launch_decompress(const Item &single, const Item *items, const uint32_t *tile_items,
                              LaunchCfg cfg, Ctrl *ctrl, uint64_t *status, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t value, hipStream_t stream);
-int max_resident_workgroups(bool compress);
+int workgroups_per_cu(bool compress, bool table, int tile);   // occupancy limit
+int num_cus();
 
 }  // namespace lsc

@@ -20,7 +20,7 @@
 
 namespace lsc {
 
-constexpr uint32_t kSpinLimit = 1u << 18;
+constexpr uint32_t kSpinLimit = 1u << 16;
 
 __device__ __forceinline__ uint64_t lb_word(uint32_t epoch, uint32_t flag, uint64_t val)
 {
@@ -52,56 +52,6 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
-}
-
-// Exclusive prefix (in words) of tile `tile` over the tiles [first, tile) of
-// its item.  Called by all 64 lanes of one wave.
-__device__ uint64_t lookback(uint64_t *status, uint32_t tile, uint32_t first, uint32_t epoch,
-                             uint32_t lane, Ctrl *ctrl)
-{
-    uint64_t excl = 0;
-    int64_t base = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    for (;;) {
-        const int64_t idx = base - (int64_t)lane;
-        const bool inr = idx >= (int64_t)first;
-        uint64_t w, incl_m, need;
-        for (;;) {
-            w = inr ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
-            const uint32_t flag = ((uint32_t)(w >> 38) == epoch) ? (uint32_t)(w >> 36) & 3u : 0u;
-            incl_m = __ballot(flag == LB_INCL);
-            const uint64_t empty_m = __ballot(flag == LB_EMPTY);
-            need = incl_m ? (((incl_m & (~incl_m + 1ull)) << 1) - 1ull) : ~0ull;
-            if (!(empty_m & need)) break;
-            // Bounded wait: after kSpinLimit polls, or once any tile of the
-            // launch has given up, stop waiting (output is then flagged bad).
-            ++spins;
-            if ((spins & 255u) == 0) {
-                const uint32_t e = __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((e & ERRF_TIMEOUT) || spins > kSpinLimit) {
-                    if (lane == 0) __hip_atomic_fetch_or(&ctrl->error, ERRF_TIMEOUT, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-                    return 0;
-                }
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        excl += wave_sum_u64(((need >> lane) & 1ull) ? (w & kValMask) : 0ull);
-        if (incl_m) return excl;
-        base -= 64;
-    }
-}
-
-// Locate the item of a global tile (binary search over first_tile).
-__device__ __forceinline__ uint32_t find_item(const Item *items, uint32_t n, uint32_t tile)
-{
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (items[mid].first_tile <= tile) lo = mid;
-        else hi = mid;
-    }
-    return lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -176,6 +126,7 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
 
 // Workgroup exclusive scan of per-lane word counts; returns the lane's offset,
 // sets *agg to the tile total.  Contains barriers.
+template <int TILE>
 __device__ __forceinline__ uint32_t tile_scan(uint32_t words, uint32_t tid, uint32_t *s_wsum,
                                               uint32_t *agg)
 {
@@ -185,7 +136,7 @@ __device__ __forceinline__ uint32_t tile_scan(uint32_t words, uint32_t tid, uint
     __syncthreads();
     uint32_t before = 0, total = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) {
+    for (int w = 0; w < TILE / 64; w++) {
         const uint32_t s = s_wsum[w];
         before += (w < (int)wave) ? s : 0u;
         total += s;
@@ -194,13 +145,79 @@ __device__ __forceinline__ uint32_t tile_scan(uint32_t words, uint32_t tid, uint
     return before + incl - words;
 }
 
+template <int TILE>
 struct TileShared {
-    uint32_t pay[kTile * 32];      // payload staging, 4-byte words (128 B per block max)
-    uint32_t hdr[kTile / 4];       // header bytes of the tile
+    static constexpr int kWaves = TILE / 64;
+    uint32_t pay[TILE * 32];       // payload staging, 4-byte words (128 B per block max)
+    uint32_t hdr[TILE / 4];        // header bytes of the tile
     uint32_t wsum[kWaves];
-    uint32_t tile, epoch, item;
-    uint32_t excl_lo, excl_hi;
+    uint64_t lb_incl[kWaves], lb_empty[kWaves], lb_sum[kWaves];
+    uint32_t lb_err;
+    uint32_t tile, epoch, ntot;
 };
+
+// Exclusive prefix (in words) of tile `tile` over the tiles [first, tile) of
+// its item: decoupled look-back with a window of TILE predecessors per probe,
+// one per thread of the workgroup.  Called by every thread (uniform control
+// flow, barriers inside).
+template <int TILE>
+__device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t first, uint32_t epoch,
+                                  TileShared<TILE> &sh, Ctrl *ctrl)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint64_t excl = 0;
+    int64_t base = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    for (;;) {
+        const int64_t idx = base - (int64_t)tid;
+        const bool inr = idx >= (int64_t)first;
+        uint64_t w;
+        uint32_t j;   // nearest inclusive predecessor, as a window position (TILE: none)
+        for (;;) {
+            w = inr ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
+            const uint32_t flag = ((uint32_t)(w >> 38) == epoch) ? (uint32_t)(w >> 36) & 3u : 0u;
+            const uint64_t incl_m = __ballot(flag == LB_INCL);
+            const uint64_t empty_m = __ballot(flag == LB_EMPTY);
+            if (lane == 0) { sh.lb_incl[wave] = incl_m; sh.lb_empty[wave] = empty_m; }
+            if (tid == 0) sh.lb_err = __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            bool retry = false;
+            j = TILE;
+#pragma unroll
+            for (int v = 0; v < TILE / 64; v++) {
+                const uint64_t im = sh.lb_incl[v], em = sh.lb_empty[v];
+                if (im) {
+                    const uint64_t need = ((im & (~im + 1ull)) << 1) - 1ull;
+                    retry = (em & need) != 0;
+                    j = 64u * v + (uint32_t)__builtin_ctzll(im);
+                    break;
+                }
+                if (em) { retry = true; break; }
+            }
+            const uint32_t err = sh.lb_err;
+            __syncthreads();
+            if (!retry) break;
+            // Bounded wait: after kSpinLimit polls, or once any tile of the
+            // launch has given up, stop waiting (the launch is flagged bad).
+            if ((err & ERRF_TIMEOUT) || ++spins > kSpinLimit) {
+                if (tid == 0) __hip_atomic_fetch_or(&ctrl->error, ERRF_TIMEOUT, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                return 0;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const uint64_t part = wave_sum_u64(tid <= j ? (w & kValMask) : 0ull);
+        if (lane == 0) sh.lb_sum[wave] = part;
+        __syncthreads();
+        uint64_t total = 0;
+#pragma unroll
+        for (int v = 0; v < TILE / 64; v++) total += sh.lb_sum[v];
+        __syncthreads();
+        excl += total;
+        if (j < (uint32_t)TILE) return excl;
+        base -= TILE;
+    }
+}
 
 __device__ __forceinline__ void block_coords(const Item &it, uint32_t b, uint32_t &bx,
                                              uint32_t &by, uint32_t &bz)
@@ -226,43 +243,47 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
     }
 }
 
-__device__ __forceinline__ void next_tile(TileShared &sh, Ctrl *ctrl)
+template <int TILE>
+__device__ __forceinline__ void next_tile(TileShared<TILE> &sh, Ctrl *ctrl)
 {
     if (threadIdx.x == 0)
         sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Item of a global tile: the single item, or items[tile_item[tile]].
 template <bool kTable>
-__device__ __forceinline__ const Item &tile_item(TileShared &sh, const Item &single, const Item *items,
-                                                 uint32_t nitems, uint32_t tile)
+__device__ __forceinline__ const Item &tile_item(const Item &single, const Item *items, const uint32_t *tile_item,
+                                                 uint32_t tile)
 {
     if (!kTable) return single;
-    if (threadIdx.x == 0) sh.item = find_item(items, nitems, tile);
-    __syncthreads();
-    return items[__builtin_amdgcn_readfirstlane(sh.item)];
+    return items[__builtin_amdgcn_readfirstlane(tile_item[tile])];
 }
 
 // ---------------------------------------------------------------------------
 // Compression.
 // ---------------------------------------------------------------------------
-template <bool kTable>
-__global__ __launch_bounds__(kTile) void compress_kernel(const Item single, const Item *__restrict__ items,
-                                                        uint32_t nitems, uint32_t total_tiles,
+template <int TILE, bool kTable>
+__global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const Item *__restrict__ items,
+                                                        const uint32_t *__restrict__ tile_items,
+                                                        uint32_t total_tiles, const uint32_t *d_total_tiles,
                                                         Ctrl *ctrl, uint64_t *status)
 {
-    __shared__ TileShared sh;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid == 0) sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ TileShared<TILE> sh;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
+    }
     next_tile(sh, ctrl);
     __syncthreads();
-    const uint32_t epoch = sh.epoch;
+    const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
     for (;;) {
         const uint32_t tile = sh.tile;
-        if (tile >= total_tiles) break;
-        const Item &it = tile_item<kTable>(sh, single, items, nitems, tile);
+        if (tile >= ntot) break;
+        const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
-        const uint32_t b = tl * (uint32_t)kTile + tid;
+        const uint32_t b = tl * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
 
         uint32_t D[32];
@@ -277,31 +298,25 @@ __global__ __launch_bounds__(kTile) void compress_kernel(const Item single, cons
         ((uint8_t *)sh.hdr)[tid] = (uint8_t)hdr;
 
         uint32_t agg;
-        const uint32_t off = tile_scan(words, tid, sh.wsum, &agg);
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
         if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
 
         if (words) encode_block(D, hdr, it.dec, &sh.pay[off]);
 
-        if (wave == 0) {
-            uint64_t excl = 0;
-            if (tl != 0) {
-                excl = lookback(status, tile, it.first_tile, epoch, lane, ctrl);
-                if (lane == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
-            }
-            if (lane == 0) { sh.excl_lo = (uint32_t)excl; sh.excl_hi = (uint32_t)(excl >> 32); }
+        uint64_t excl = 0;
+        if (tl != 0) {
+            excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh, ctrl);
+            if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
         }
-        __syncthreads();
-
-        const uint64_t excl = ((uint64_t)sh.excl_hi << 32) | sh.excl_lo;
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
-        // header bytes [tl*kTile, tl*kTile + kTile) clipped to R (pad bytes are 0)
-        for (uint32_t w = tid; w < (uint32_t)kTile / 4; w += kTile) {
-            const uint64_t o = (uint64_t)tl * kTile + 4ull * w;
+        // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0)
+        for (uint32_t w = tid; w < (uint32_t)TILE / 4; w += TILE) {
+            const uint64_t o = (uint64_t)tl * TILE + 4ull * w;
             if (o < it.hdr_bytes) *(uint32_t *)(it.s_out + o) = sh.hdr[w];
         }
         if (g0 + 4ull * agg <= it.s_len) {
             uint32_t *dst = (uint32_t *)(it.s_out + g0);
-            for (uint32_t w = tid; w < agg; w += kTile) dst[w] = sh.pay[w];
+            for (uint32_t w = tid; w < agg; w += TILE) dst[w] = sh.pay[w];
         }
         if (tid == 0 && tl == it.ntiles - 1 && it.result) *it.result = g0 + 4ull * agg;
 
@@ -315,24 +330,28 @@ __global__ __launch_bounds__(kTile) void compress_kernel(const Item single, cons
 // ---------------------------------------------------------------------------
 // Decompression.
 // ---------------------------------------------------------------------------
-template <bool kTable>
-__global__ __launch_bounds__(kTile) void decompress_kernel(const Item single, const Item *__restrict__ items,
-                                                          uint32_t nitems, uint32_t total_tiles,
+template <int TILE, bool kTable>
+__global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, const Item *__restrict__ items,
+                                                        const uint32_t *__restrict__ tile_items,
+                                                          uint32_t total_tiles, const uint32_t *d_total_tiles,
                                                           Ctrl *ctrl, uint64_t *status)
 {
-    __shared__ TileShared sh;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid == 0) sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ TileShared<TILE> sh;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
+    }
     next_tile(sh, ctrl);
     __syncthreads();
-    const uint32_t epoch = sh.epoch;
+    const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
     for (;;) {
         const uint32_t tile = sh.tile;
-        if (tile >= total_tiles) break;
-        const Item &it = tile_item<kTable>(sh, single, items, nitems, tile);
+        if (tile >= ntot) break;
+        const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
-        const uint32_t b = tl * (uint32_t)kTile + tid;
+        const uint32_t b = tl * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
 
         const uint32_t hdr = valid ? (uint32_t)it.s_in[b] : 0u;
@@ -340,24 +359,18 @@ __global__ __launch_bounds__(kTile) void decompress_kernel(const Item single, co
         bool bad = (hdr & 0x1fu) > 16u;
 
         uint32_t agg;
-        const uint32_t off = tile_scan(words, tid, sh.wsum, &agg);
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
         if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
-        if (wave == 0) {
-            uint64_t excl = 0;
-            if (tl != 0) {
-                excl = lookback(status, tile, it.first_tile, epoch, lane, ctrl);
-                if (lane == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
-            }
-            if (lane == 0) { sh.excl_lo = (uint32_t)excl; sh.excl_hi = (uint32_t)(excl >> 32); }
+        uint64_t excl = 0;
+        if (tl != 0) {
+            excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh, ctrl);
+            if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
         }
-        __syncthreads();
-
-        const uint64_t excl = ((uint64_t)sh.excl_hi << 32) | sh.excl_lo;
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
-        const bool fits = agg <= (uint32_t)kTile * 32u && g0 + 4ull * agg <= it.s_len;
+        const bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
         if (fits) {
             const uint32_t *src = (const uint32_t *)(it.s_in + g0);
-            for (uint32_t w = tid; w < agg; w += kTile) sh.pay[w] = src[w];
+            for (uint32_t w = tid; w < agg; w += TILE) sh.pay[w] = src[w];
         }
         __syncthreads();
 
@@ -390,28 +403,50 @@ __global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t value)
         p[i] = value;
 }
 
-hipError_t launch_compress(const Item &single, const Item *items, uint32_t nitems, LaunchCfg cfg,
+template <int TILE>
+static hipError_t launch_t(bool compress, const Item &single, const Item *items, const uint32_t *tile_items, LaunchCfg cfg,
                            Ctrl *ctrl, uint64_t *status, hipStream_t stream)
 {
-    if (items)
-        hipLaunchKernelGGL(compress_kernel<true>, dim3(cfg.grid), dim3(kTile), 0, stream, single, items,
-                           nitems, cfg.total_tiles, ctrl, status);
-    else
-        hipLaunchKernelGGL(compress_kernel<false>, dim3(cfg.grid), dim3(kTile), 0, stream, single, items,
-                           nitems, cfg.total_tiles, ctrl, status);
+    const dim3 grid(cfg.grid), block(TILE);
+    if (compress) {
+        if (items)
+            hipLaunchKernelGGL((compress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+        else
+            hipLaunchKernelGGL((compress_kernel<TILE, false>), grid, block, 0, stream, single, items, tile_items,
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+    } else {
+        if (items)
+            hipLaunchKernelGGL((decompress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+        else
+            hipLaunchKernelGGL((decompress_kernel<TILE, false>), grid, block, 0, stream, single, items, tile_items,
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_decompress(const Item &single, const Item *items, uint32_t nitems, LaunchCfg cfg,
-                             Ctrl *ctrl, uint64_t *status, hipStream_t stream)
+static hipError_t launch(bool compress, const Item &single, const Item *items, const uint32_t *tile_items, LaunchCfg cfg,
+                         Ctrl *ctrl, uint64_t *status, hipStream_t stream)
 {
-    if (items)
-        hipLaunchKernelGGL(decompress_kernel<true>, dim3(cfg.grid), dim3(kTile), 0, stream, single, items,
-                           nitems, cfg.total_tiles, ctrl, status);
-    else
-        hipLaunchKernelGGL(decompress_kernel<false>, dim3(cfg.grid), dim3(kTile), 0, stream, single, items,
-                           nitems, cfg.total_tiles, ctrl, status);
-    return hipGetLastError();
+    switch (cfg.tile) {
+    case 128: return launch_t<128>(compress, single, items, tile_items, cfg, ctrl, status, stream);
+    case 256: return launch_t<256>(compress, single, items, tile_items, cfg, ctrl, status, stream);
+    case 512: return launch_t<512>(compress, single, items, tile_items, cfg, ctrl, status, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_compress(const Item &single, const Item *items, const uint32_t *tile_items, LaunchCfg cfg, Ctrl *ctrl,
+                           uint64_t *status, hipStream_t stream)
+{
+    return launch(true, single, items, tile_items, cfg, ctrl, status, stream);
+}
+
+hipError_t launch_decompress(const Item &single, const Item *items, const uint32_t *tile_items, LaunchCfg cfg, Ctrl *ctrl,
+                             uint64_t *status, hipStream_t stream)
+{
+    return launch(false, single, items, tile_items, cfg, ctrl, status, stream);
 }
 
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t value, hipStream_t stream)
@@ -423,16 +458,36 @@ hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t value, hipStream_t 
     return hipGetLastError();
 }
 
-int max_resident_workgroups(bool compress)
+template <int TILE>
+static int per_cu_t(bool compress, bool table)
 {
-    int dev = 0, ncu = 0, per_cu = 0;
+    int per_cu = 0;
+    hipError_t e;
+    if (compress)
+        e = table ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, compress_kernel<TILE, true>, TILE, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, compress_kernel<TILE, false>, TILE, 0);
+    else
+        e = table ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_kernel<TILE, true>, TILE, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_kernel<TILE, false>, TILE, 0);
+    return (e == hipSuccess && per_cu > 0) ? per_cu : 1;
+}
+
+int workgroups_per_cu(bool compress, bool table, int tile)
+{
+    switch (tile) {
+    case 128: return per_cu_t<128>(compress, table);
+    case 256: return per_cu_t<256>(compress, table);
+    case 512: return per_cu_t<512>(compress, table);
+    default: return 1;
+    }
+}
+
+int num_cus()
+{
+    int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    hipError_t e = compress
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, compress_kernel<false>, kTile, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_kernel<false>, kTile, 0);
-    if (e != hipSuccess || per_cu < 1) per_cu = 1;
-    return ncu * per_cu;
+    return ncu;
 }
 
 }  // namespace lsc

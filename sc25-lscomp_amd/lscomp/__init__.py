@@ -46,12 +46,21 @@ class Bins4(ctypes.Structure):
                 ("w", ctypes.c_uint32)]
 
 
+class ItemDesc(ctypes.Structure):
+    """lscomp_item_u16."""
+    _fields_ = [("dims", Dim3), ("bins", Bins4), ("pooling_th", ctypes.c_float),
+                ("d_volume", ctypes.c_void_p), ("d_stream", ctypes.c_void_p), ("stream_bytes", ctypes.c_uint64)]
+
+
 # Every symbol include/lscomp.h declares.
 EXPORTS = (
     "lscomp_num_blocks_u16", "lscomp_header_bytes_u16", "lscomp_max_compressed_bytes_u16",
     "lscomp_compress_u16", "lscomp_compress_u16_async", "lscomp_decompress_u16",
     "lscomp_decompress_u16_async", "lscomp_stream_status", "lscomp_reserve_u16",
-    "lscomp_release_all", "lscomp_status_string", "lscomp_version",
+    "lscomp_release_all", "lscomp_set_launch_shape", "lscomp_get_launch_shape", "lscomp_status_string",
+    "lscomp_version", "lscomp_plan_create", "lscomp_plan_set_items_u16", "lscomp_plan_launch",
+    "lscomp_plan_launch_graph", "lscomp_plan_device_sizes", "lscomp_plan_sizes", "lscomp_plan_status",
+    "lscomp_plan_destroy",
 )
 
 
@@ -77,6 +86,24 @@ def _load():
     lib.lscomp_reserve_u16.argtypes = [Dim3, vp]
     lib.lscomp_release_all.restype = None
     lib.lscomp_release_all.argtypes = []
+    lib.lscomp_set_launch_shape.restype = ctypes.c_int
+    lib.lscomp_set_launch_shape.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+    lib.lscomp_get_launch_shape.restype = ctypes.c_int
+    lib.lscomp_get_launch_shape.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(ctypes.c_uint32)]
+    lib.lscomp_plan_create.restype = ctypes.c_int
+    lib.lscomp_plan_create.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(vp)]
+    lib.lscomp_plan_set_items_u16.restype = ctypes.c_int
+    lib.lscomp_plan_set_items_u16.argtypes = [vp, ctypes.POINTER(ItemDesc), ctypes.c_uint32, vp]
+    for name in ("lscomp_plan_launch", "lscomp_plan_launch_graph", "lscomp_plan_status"):
+        getattr(lib, name).restype = ctypes.c_int
+        getattr(lib, name).argtypes = [vp, vp]
+    lib.lscomp_plan_device_sizes.restype = vp
+    lib.lscomp_plan_device_sizes.argtypes = [vp]
+    lib.lscomp_plan_sizes.restype = ctypes.c_int
+    lib.lscomp_plan_sizes.argtypes = [vp, u64p, vp]
+    lib.lscomp_plan_destroy.restype = ctypes.c_int
+    lib.lscomp_plan_destroy.argtypes = [vp]
     lib.lscomp_status_string.restype = ctypes.c_char_p
     lib.lscomp_status_string.argtypes = [ctypes.c_int]
     lib.lscomp_version.restype = ctypes.c_int
@@ -197,5 +224,80 @@ def release_all() -> None:
     _lib.lscomp_release_all()
 
 
+COMPRESS, DECOMPRESS = 0, 1
+
+
+def set_launch_shape(direction: int, tile: int = 0, wg_per_cu: int = 0) -> None:
+    """Blocks per workgroup (128/256/512, 0 = default) and workgroups per CU (0 = occupancy limit)."""
+    st = _lib.lscomp_set_launch_shape(direction, tile, wg_per_cu)
+    if st != OK:
+        raise LscompError(st, "lscomp_set_launch_shape")
+
+
+def get_launch_shape(direction: int):
+    t, w = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    st = _lib.lscomp_get_launch_shape(direction, ctypes.byref(t), ctypes.byref(w))
+    if st != OK:
+        raise LscompError(st, "lscomp_get_launch_shape")
+    return int(t.value), int(w.value)
+
+
 def version() -> int:
     return int(_lib.lscomp_version())
+
+
+class Plan:
+    """Batched form: many independent items (own streams) per persistent launch.
+
+    ``items`` for ``set_items`` are tuples (volume tensor, dims, bins, pooling,
+    stream tensor, stream_bytes): for compression the stream tensor is the output
+    buffer and stream_bytes its capacity; for decompression the input stream and
+    its length.  Volumes must stay alive until the launch finished.
+    """
+
+    def __init__(self, direction: int, max_items: int, max_blocks: int):
+        h = ctypes.c_void_p()
+        st = _lib.lscomp_plan_create(direction, max_items, max_blocks, ctypes.byref(h))
+        if st != OK:
+            raise LscompError(st, "lscomp_plan_create")
+        self.h, self.direction, self.n = h, direction, 0
+
+    def set_items(self, items, stream=None) -> None:
+        arr = (ItemDesc * len(items))()
+        for i, (vol, dims, bins, pooling, sbuf, sbytes) in enumerate(items):
+            _check_volume(vol, dims)
+            arr[i] = ItemDesc(_dims(dims), _bins(bins), float(pooling), vol.data_ptr(), sbuf.data_ptr(), int(sbytes))
+        st = _lib.lscomp_plan_set_items_u16(self.h, arr, len(items), _stream_handle(stream))
+        if st != OK:
+            raise LscompError(st, "lscomp_plan_set_items_u16")
+        self.n = len(items)
+
+    def launch(self, stream=None, graph: bool = False) -> None:
+        f = _lib.lscomp_plan_launch_graph if graph else _lib.lscomp_plan_launch
+        st = f(self.h, _stream_handle(stream))
+        if st != OK:
+            raise LscompError(st, "lscomp_plan_launch")
+
+    def sizes(self, stream=None):
+        out = (ctypes.c_uint64 * max(self.n, 1))()
+        st = _lib.lscomp_plan_sizes(self.h, out, _stream_handle(stream))
+        if st != OK:
+            raise LscompError(st, "lscomp_plan_sizes")
+        return [int(v) for v in out[: self.n]]
+
+    def device_sizes_ptr(self) -> int:
+        return int(_lib.lscomp_plan_device_sizes(self.h) or 0)
+
+    def status(self, stream=None) -> int:
+        return int(_lib.lscomp_plan_status(self.h, _stream_handle(stream)))
+
+    def close(self) -> None:
+        if self.h:
+            _lib.lscomp_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

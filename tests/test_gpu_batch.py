@@ -1,0 +1,85 @@
+"""Batched form on the GPU: one persistent launch over many uneven items gives,
+for every item, exactly the stream of a plain call on that item (and the
+oracle's), also when replayed from a hipGraph and when the plan is refilled."""
+import numpy as np
+import pytest
+
+from lsc_helpers import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames(seed, n, sides=(8, 24, 64, 136, 256, 512)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        y = int(rng.choice(sides)) + int(rng.integers(0, 3))
+        z = int(rng.choice(sides)) + int(rng.integers(0, 2)) * 2
+        x = int(rng.integers(1, 3))
+        out.append(synth("xpcs" if i % 2 else "cssi", (x, y, z), seed=seed * 1000 + i))
+    return out
+
+
+def _compress_batch(gpu, vols, params, graph=False, plan=None):
+    import torch
+    dev = [torch.from_numpy(v.view(np.int16)).cuda() for v in vols]
+    outs = [torch.zeros(gpu.max_compressed_bytes(v.shape), dtype=torch.uint8, device="cuda") for v in vols]
+    if plan is None:
+        plan = gpu.Plan(gpu.COMPRESS, len(vols), sum(gpu.num_blocks(v.shape) for v in vols))
+    plan.set_items([(d, v.shape, b, p, o, o.numel()) for d, v, (b, p), o in zip(dev, vols, params, outs)])
+    plan.launch(graph=graph)
+    sizes = plan.sizes()
+    assert plan.status() == gpu.OK
+    return plan, [o[:n].cpu().numpy() for o, n in zip(outs, sizes)]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_oracle_and_plain(gpu, orc, graph):
+    vols = _frames(1, 24)
+    params = [((1, 1, 1, 1), 1.0) if i % 2 == 0 else ((3, 5, 10, 15), 0.5) for i in range(len(vols))]
+    plan, streams = _compress_batch(gpu, vols, params, graph=graph)
+    for v, (b, p), s in zip(vols, params, streams):
+        st, ref, refn = orc.compress(v, b, p)
+        assert s.size == refn and np.array_equal(s, ref)
+    # decompress the same streams as one batch
+    import torch
+    dplan = gpu.Plan(gpu.DECOMPRESS, len(vols), sum(gpu.num_blocks(v.shape) for v in vols))
+    dsrc = [torch.from_numpy(s.copy()).cuda() for s in streams]
+    dout = [torch.full(v.shape, 7, dtype=torch.int16, device="cuda") for v in vols]
+    dplan.set_items([(o, v.shape, b, p, s, s.numel()) for o, v, (b, p), s in zip(dout, vols, params, dsrc)])
+    dplan.launch(graph=graph)
+    assert dplan.status() == gpu.OK
+    for o, v, (b, p), s in zip(dout, vols, params, streams):
+        _, refdec = orc.decompress(s, v.shape, b)
+        assert np.array_equal(o.cpu().numpy().view(np.uint16), refdec)
+    plan.close()
+    dplan.close()
+
+
+def test_plan_refill_and_graph_replay(gpu, orc):
+    """One plan, several batches; the captured graph picks up each new batch."""
+    vols_all = _frames(2, 30)
+    maxb = sum(gpu.num_blocks(v.shape) for v in vols_all)
+    plan = gpu.Plan(gpu.COMPRESS, 30, maxb)
+    for rep, (lo, hi) in enumerate([(0, 10), (10, 30), (5, 12), (0, 30)]):
+        vols = vols_all[lo:hi]
+        params = [((3, 5, 10, 15), 0.5)] * len(vols)
+        _, streams = _compress_batch(gpu, vols, params, graph=True, plan=plan)
+        for v, s in zip(vols, streams):
+            st, ref, refn = orc.compress(v, (3, 5, 10, 15), 0.5)
+            assert np.array_equal(s, ref), (rep, v.shape)
+    plan.close()
+
+
+def test_plan_rejects_bad_items(gpu):
+    import torch
+    plan = gpu.Plan(gpu.COMPRESS, 2, 64)
+    v = torch.zeros((1, 8, 8), dtype=torch.int16, device="cuda")
+    o = torch.zeros(256, dtype=torch.uint8, device="cuda")
+    with pytest.raises(gpu.LscompError) as ei:
+        plan.set_items([(v, (1, 8, 8), (3, 2, 5, 6), 0.5, o, 256)])
+    assert ei.value.status == gpu.ERR_INVALID_ARG
+    big = torch.zeros((1, 1024, 1024), dtype=torch.int16, device="cuda")
+    with pytest.raises(gpu.LscompError) as ei:      # more blocks than the plan holds
+        plan.set_items([(big, (1, 1024, 1024), (1, 1, 1, 1), 1.0, o, 256)])
+    plan.close()

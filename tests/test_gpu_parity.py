@@ -164,3 +164,18 @@ def test_decode_oracle_stream(gpu, orc):
         dec = gpu.decompress(torch.from_numpy(ref.copy()).cuda(), refn, vol.shape, bins)
         _, refdec = orc.decompress(ref, vol.shape, bins)
         assert np.array_equal(dec.cpu().numpy().view(np.uint16), refdec)
+
+
+@pytest.mark.parametrize("tile", [128, 256, 512])
+@pytest.mark.parametrize("wg_per_cu", [0, 1])
+def test_launch_shapes_give_identical_bytes(gpu, orc, tile, wg_per_cu):
+    """The stream does not depend on the launch shape (tile size, persistent grid size)."""
+    vol = synth("cssi", (6, 300, 1558), seed=13)
+    try:
+        for d in (gpu.COMPRESS, gpu.DECOMPRESS):
+            gpu.set_launch_shape(d, tile, wg_per_cu)
+        check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+        check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
+    finally:
+        for d in (gpu.COMPRESS, gpu.DECOMPRESS):
+            gpu.set_launch_shape(d, 0, 0)
