@@ -148,6 +148,53 @@ int lscomp_plan_sizes(lscomp_plan plan, uint64_t *host_sizes, hipStream_t stream
 int lscomp_plan_status(lscomp_plan plan, hipStream_t stream);
 int lscomp_plan_destroy(lscomp_plan plan);
 
+/* ---- Served form --------------------------------------------------------
+ * A compression server with continuous batching.  Requests (device frames)
+ * are queued by lscomp_server_submit_u16; a scheduler thread admits them in
+ * arrival order while their worst-case output fits into an HBM arena of
+ * `budget_bytes`, and packs every admitted request into the next batch of
+ * one of `slots` plans, each on its own stream, so batches started while
+ * others still run overlap on the device.  Each request's stream is exactly
+ * lscomp_compress_u16 of its frame alone.  Output buffers stay reserved until
+ * lscomp_server_release.  No counterpart in the reference. */
+typedef struct lscomp_request_u16 {
+    lscomp_dim3 dims;
+    lscomp_bins4 bins;
+    float pooling_th;
+    const uint16_t *d_frame;   /* device input, alive until the request completes */
+    uint64_t tag;              /* returned with the completion */
+} lscomp_request_u16;
+
+typedef struct lscomp_completion {
+    uint64_t id;               /* from lscomp_server_submit_u16 */
+    uint64_t tag;
+    int status;                /* LSCOMP_OK, or an error for this request */
+    uint32_t reserved;
+    uint64_t stream_bytes;
+    const uint8_t *d_stream;   /* in the server's arena; valid until released */
+    double t_submit, t_admit, t_launch, t_done;   /* seconds (steady clock) */
+} lscomp_completion;
+
+typedef struct lscomp_server_s *lscomp_server;
+
+/* slots: batches in flight (streams); max_items / max_blocks: per batch. */
+int lscomp_server_create(uint32_t slots, uint32_t max_items, uint64_t max_blocks, uint64_t budget_bytes,
+                         lscomp_server *srv);
+int lscomp_server_submit_u16(lscomp_server srv, const lscomp_request_u16 *req, uint64_t *id);
+/* Non-blocking: copies up to max finished requests to out, sets *n. */
+int lscomp_server_poll(lscomp_server srv, lscomp_completion *out, uint32_t max, uint32_t *n);
+/* Frees the output buffer of a finished request (makes room for admission). */
+int lscomp_server_release(lscomp_server srv, uint64_t id);
+/* Blocks until every submitted request has finished. */
+int lscomp_server_drain(lscomp_server srv);
+/* Submits reqs[i] at time t0 + arrival_s[i] (busy-waiting on the calling
+ * thread, t0 = call time), collects every completion into out[i] (by request
+ * order) and releases each output as soon as it is copied out; returns when
+ * all finished.  A trace replay for load tests (e.g. Poisson arrivals). */
+int lscomp_server_replay_u16(lscomp_server srv, const lscomp_request_u16 *reqs, const double *arrival_s,
+                             uint32_t n, lscomp_completion *out);
+int lscomp_server_destroy(lscomp_server srv);
+
 const char *lscomp_status_string(int status);
 
 /* Library version, major*10000 + minor*100 + patch. */

@@ -60,8 +60,23 @@ EXPORTS = (
     "lscomp_release_all", "lscomp_set_launch_shape", "lscomp_get_launch_shape", "lscomp_status_string",
     "lscomp_version", "lscomp_plan_create", "lscomp_plan_set_items_u16", "lscomp_plan_launch",
     "lscomp_plan_launch_graph", "lscomp_plan_device_sizes", "lscomp_plan_sizes", "lscomp_plan_status",
-    "lscomp_plan_destroy",
+    "lscomp_plan_destroy", "lscomp_server_create", "lscomp_server_submit_u16", "lscomp_server_poll",
+    "lscomp_server_release", "lscomp_server_drain", "lscomp_server_replay_u16", "lscomp_server_destroy",
 )
+
+
+class Request(ctypes.Structure):
+    """lscomp_request_u16."""
+    _fields_ = [("dims", Dim3), ("bins", Bins4), ("pooling_th", ctypes.c_float),
+                ("d_frame", ctypes.c_void_p), ("tag", ctypes.c_uint64)]
+
+
+class Completion(ctypes.Structure):
+    """lscomp_completion."""
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("status", ctypes.c_int),
+                ("reserved", ctypes.c_uint32), ("stream_bytes", ctypes.c_uint64), ("d_stream", ctypes.c_void_p),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_launch", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
 
 
 def _load():
@@ -104,6 +119,23 @@ def _load():
     lib.lscomp_plan_sizes.argtypes = [vp, u64p, vp]
     lib.lscomp_plan_destroy.restype = ctypes.c_int
     lib.lscomp_plan_destroy.argtypes = [vp]
+    lib.lscomp_server_create.restype = ctypes.c_int
+    lib.lscomp_server_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
+                                         ctypes.POINTER(vp)]
+    lib.lscomp_server_submit_u16.restype = ctypes.c_int
+    lib.lscomp_server_submit_u16.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    lib.lscomp_server_poll.restype = ctypes.c_int
+    lib.lscomp_server_poll.argtypes = [vp, ctypes.POINTER(Completion), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint32)]
+    lib.lscomp_server_release.restype = ctypes.c_int
+    lib.lscomp_server_release.argtypes = [vp, ctypes.c_uint64]
+    lib.lscomp_server_drain.restype = ctypes.c_int
+    lib.lscomp_server_drain.argtypes = [vp]
+    lib.lscomp_server_replay_u16.restype = ctypes.c_int
+    lib.lscomp_server_replay_u16.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double),
+                                             ctypes.c_uint32, ctypes.POINTER(Completion)]
+    lib.lscomp_server_destroy.restype = ctypes.c_int
+    lib.lscomp_server_destroy.argtypes = [vp]
     lib.lscomp_status_string.restype = ctypes.c_char_p
     lib.lscomp_status_string.argtypes = [ctypes.c_int]
     lib.lscomp_version.restype = ctypes.c_int
@@ -294,6 +326,67 @@ class Plan:
     def close(self) -> None:
         if self.h:
             _lib.lscomp_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Served form: continuous batching of frame-compression requests (see include/lscomp.h)."""
+
+    def __init__(self, slots: int, max_items: int, max_blocks: int, budget_bytes: int):
+        h = ctypes.c_void_p()
+        st = _lib.lscomp_server_create(slots, max_items, max_blocks, budget_bytes, ctypes.byref(h))
+        if st != OK:
+            raise LscompError(st, "lscomp_server_create")
+        self.h = h
+
+    @staticmethod
+    def request(frame, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, tag: int = 0) -> Request:
+        _check_volume(frame, dims)
+        return Request(_dims(dims), _bins(bins), float(pooling), frame.data_ptr(), int(tag))
+
+    def submit(self, req: Request) -> int:
+        rid = ctypes.c_uint64(0)
+        st = _lib.lscomp_server_submit_u16(self.h, ctypes.byref(req), ctypes.byref(rid))
+        if st != OK:
+            raise LscompError(st, "lscomp_server_submit_u16")
+        return int(rid.value)
+
+    def poll(self, max_n: int = 256):
+        buf = (Completion * max_n)()
+        n = ctypes.c_uint32(0)
+        st = _lib.lscomp_server_poll(self.h, buf, max_n, ctypes.byref(n))
+        if st != OK:
+            raise LscompError(st, "lscomp_server_poll")
+        return [buf[i] for i in range(n.value)]
+
+    def release(self, rid: int) -> None:
+        st = _lib.lscomp_server_release(self.h, rid)
+        if st != OK:
+            raise LscompError(st, "lscomp_server_release")
+
+    def drain(self) -> int:
+        return int(_lib.lscomp_server_drain(self.h))
+
+    def replay(self, reqs, arrivals):
+        """Submit reqs at the given arrival times (s); returns Completion per request, in order."""
+        n = len(reqs)
+        arr = (Request * n)(*reqs)
+        times = (ctypes.c_double * n)(*[float(t) for t in arrivals])
+        out = (Completion * n)()
+        st = _lib.lscomp_server_replay_u16(self.h, arr, times, n, out)
+        if st != OK:
+            raise LscompError(st, "lscomp_server_replay_u16")
+        return [out[i] for i in range(n)]
+
+    def close(self) -> None:
+        if self.h:
+            _lib.lscomp_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
