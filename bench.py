@@ -64,6 +64,79 @@ def cpu_baseline(vol, sample_frames, bins, pooling):
     return sub.nbytes / dt / 1e9, orc.max_threads(), sub.nbytes, dt
 
 
+def bench_batched(torch, lscomp, datagen, np, steps):
+    """512 square frames (sides drawn from 256..2048, XPCS-style, seed 4): one persistent
+    plan launch (replayed from a hipGraph) vs 512 plain per-frame launches; identical bytes."""
+    rng = np.random.default_rng(4)
+    sides = rng.choice([256, 512, 768, 1024, 1536, 2048], size=512)
+    frames = [torch.from_numpy(datagen.volume("xpcs", 4, (1, int(s), int(s)), first_frame=i).view(np.int16)).cuda()
+              for i, s in enumerate(sides)]
+    dims = [(1, int(s), int(s)) for s in sides]
+    caps = [lscomp.max_compressed_bytes(d) for d in dims]
+    bins, pool = (1, 1, 1, 1), 1.0
+    out_plain = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    out_batch = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    sizes = torch.zeros(len(frames), dtype=torch.int64, device="cuda")
+    in_bytes = sum(2 * d[1] * d[2] for d in dims)
+    nblocks = sum(lscomp.num_blocks(d) for d in dims)
+    lscomp.reserve(max(dims, key=lambda d: d[1]))
+
+    def plain():
+        for i, (f, d) in enumerate(zip(frames, dims)):
+            lscomp.compress_async(f, d, bins, pool, out_plain[i], sizes[i:i + 1])
+
+    plan = lscomp.Plan(lscomp.COMPRESS, len(frames), nblocks)
+    plan.set_items([(f, d, bins, pool, o, o.numel()) for f, d, o in zip(frames, dims, out_batch)])
+    batch = lambda: plan.launch(graph=True)
+    for fn in (plain, batch):
+        fn()
+    torch.cuda.synchronize()
+    _, plain_ms, _ = timed_loop(torch, plain, steps)
+    _, batch_ms, _ = timed_loop(torch, batch, steps)
+    bsz = plan.sizes()
+    psz = sizes.cpu().tolist()
+    same = bsz == psz and all(torch.equal(a[:n], b[:n]) for a, b, n in zip(out_plain, out_batch, bsz))
+    ok = plan.status() == lscomp.OK and lscomp.stream_status() == lscomp.OK
+    plan.close()
+    return {"frames": len(frames), "input_bytes": in_bytes,
+            "plain_launches_ms": plain_ms, "plain_frames_per_s": len(frames) / (plain_ms * 1e-3),
+            "batched_graph_ms": batch_ms, "batched_frames_per_s": len(frames) / (batch_ms * 1e-3),
+            "batched_GBps": in_bytes / (batch_ms * 1e-3) / 1e9, "speedup": plain_ms / batch_ms,
+            "identical_streams": bool(same), "verified": bool(ok)}
+
+
+def bench_served(torch, lscomp, datagen, np, seconds, rate):
+    """Poisson arrivals at `rate` frames/s for `seconds`: frames 512^2..2048^2 CSSI-style,
+    half lossless, half bins 3 5 10 15 / pooling 0.5, admitted by a 64 GB HBM budget."""
+    rng = np.random.default_rng(5)
+    pool_sides = [512, 768, 1024, 1536, 2048]
+    pool = []
+    for i in range(40):
+        s = pool_sides[i % len(pool_sides)]
+        v = datagen.volume("cssi", 5, (1, s, s), first_frame=i)
+        pool.append((torch.from_numpy(v.view(np.int16)).cuda(), (1, s, s)))
+    n = int(seconds * rate)
+    pick = rng.integers(0, len(pool), n)
+    reqs = []
+    for i, k in enumerate(pick):
+        f, d = pool[k]
+        b, p = ((1, 1, 1, 1), 1.0) if i % 2 == 0 else ((3, 5, 10, 15), 0.5)
+        reqs.append(lscomp.Server.request(f, d, b, p, tag=i))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    srv = lscomp.Server(slots=4, max_items=256, max_blocks=1 << 22, budget_bytes=64 << 30)
+    t0 = time.perf_counter()
+    out = srv.replay(reqs, arrivals)
+    wall = time.perf_counter() - t0
+    ok = srv.drain() == lscomp.OK and all(c.status == lscomp.OK for c in out)
+    srv.close()
+    lat = np.array([c.t_done - c.t_submit for c in out]) * 1e3
+    byts = sum(2 * pool[k][1][1] * pool[k][1][2] for k in pick)
+    return {"offered_frames_per_s": rate, "requests": n, "frames_per_s": n / wall,
+            "GBps": byts / wall / 1e9, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "budget_bytes": 64 << 30, "slots": 4, "verified": bool(ok)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +145,9 @@ def main():
     ap.add_argument("--frames", type=int, default=CSSI_DIMS[0], help="frames of the CSSI volume (default 600)")
     ap.add_argument("--cpu-frames", type=int, default=48, help="frames of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--served-seconds", type=float, default=2.0)
+    ap.add_argument("--served-rate", type=float, default=20000.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; use --gpus 1", file=sys.stderr)
@@ -137,6 +213,14 @@ def main():
     copy_gbps = 2 * in_bytes / (cp_ms * 1e-3) / 1e9
     del cp
 
+    forms = {}
+    if not args.no_forms:
+        del d_dec
+        forms["batched"] = bench_batched(torch, lscomp, datagen, np, max(3, args.steps // 2))
+        log(f"batched: {json.dumps(forms['batched'])}")
+        forms["served"] = bench_served(torch, lscomp, datagen, np, args.served_seconds, args.served_rate)
+        log(f"served: {json.dumps(forms['served'])}")
+
     ll = results["lossless"]
     s_bytes = ll["stream_bytes"]
     achieved = (in_bytes + s_bytes) / (ll["compress_kernel_ms"] * 1e-3)
@@ -159,6 +243,7 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": None},
         "cpu_baseline": cpu,
         "modes": results,
+        "forms": forms,
         "device_copy_GBps": copy_gbps,
     }
     print(json.dumps(rec), flush=True)

@@ -107,6 +107,26 @@ void lscomp_release_all(void);
 int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu);
 int lscomp_get_launch_shape(int direction, uint32_t *tile, uint32_t *wg_per_cu);
 
+/* One measured launch shape (times in ms, averaged over the repetitions). */
+typedef struct lscomp_tune_entry {
+    uint32_t tile, wg_per_cu;
+    float compress_ms, decompress_ms;
+} lscomp_tune_entry;
+
+/* Autotuner: times compression and decompression of the caller's volume
+ * (d_in, dims, bins, pooling) for every launch shape (tile 128/256/512 x
+ * workgroups per CU 1..4 and the occupancy limit), using d_stream (capacity
+ * >= lscomp_max_compressed_bytes_u16(dims)) and d_decoded (same size as the
+ * volume) as work buffers, then applies the fastest shape of each direction
+ * with lscomp_set_launch_shape.  Results are cached per (size class, mode):
+ * a later call for the same class applies the cached shapes without timing.
+ * If table/n_table are given, the measured entries are copied there (up to
+ * *n_table; *n_table is set to the number written, 0 for a cache hit).
+ * No counterpart in the reference. */
+int lscomp_autotune_u16(const uint16_t *d_in, uint8_t *d_stream, size_t stream_capacity, uint16_t *d_decoded,
+                        lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, int reps, hipStream_t stream,
+                        lscomp_tune_entry *table, uint32_t *n_table);
+
 /* ---- Batched form -------------------------------------------------------
  * Many independent volumes (e.g. detector frames of uneven sizes), each with
  * its own stream in exactly the plain format, processed by ONE persistent

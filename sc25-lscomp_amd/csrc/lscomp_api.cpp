@@ -348,8 +348,7 @@ int lscomp_stream_status(hipStream_t stream)
 int lscomp_reserve_u16(lscomp_dim3 dims, hipStream_t stream)
 {
     Item it;
-    const uint32_t tmin = g_tile[0].load() < g_tile[1].load() ? g_tile[0].load() : g_tile[1].load();
-    int st = make_item(dims, lscomp_bins4{1, 1, 1, 1}, 1.0f, it, tmin);
+    int st = make_item(dims, lscomp_bins4{1, 1, 1, 1}, 1.0f, it, 128);   // smallest tile: most words
     if (st) return st;
     Scratch *s = get_scratch(stream);
     if (!s) return LSCOMP_ERR_HIP;

@@ -62,7 +62,14 @@ EXPORTS = (
     "lscomp_plan_launch_graph", "lscomp_plan_device_sizes", "lscomp_plan_sizes", "lscomp_plan_status",
     "lscomp_plan_destroy", "lscomp_server_create", "lscomp_server_submit_u16", "lscomp_server_poll",
     "lscomp_server_release", "lscomp_server_drain", "lscomp_server_replay_u16", "lscomp_server_destroy",
+    "lscomp_autotune_u16",
 )
+
+
+class TuneEntry(ctypes.Structure):
+    """lscomp_tune_entry."""
+    _fields_ = [("tile", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("compress_ms", ctypes.c_float),
+                ("decompress_ms", ctypes.c_float)]
 
 
 class Request(ctypes.Structure):
@@ -136,6 +143,9 @@ def _load():
                                              ctypes.c_uint32, ctypes.POINTER(Completion)]
     lib.lscomp_server_destroy.restype = ctypes.c_int
     lib.lscomp_server_destroy.argtypes = [vp]
+    lib.lscomp_autotune_u16.restype = ctypes.c_int
+    lib.lscomp_autotune_u16.argtypes = [vp, vp, sz, vp, Dim3, Bins4, ctypes.c_float, ctypes.c_int, vp,
+                                        ctypes.POINTER(TuneEntry), ctypes.POINTER(ctypes.c_uint32)]
     lib.lscomp_status_string.restype = ctypes.c_char_p
     lib.lscomp_status_string.argtypes = [ctypes.c_int]
     lib.lscomp_version.restype = ctypes.c_int
@@ -272,6 +282,27 @@ def get_launch_shape(direction: int):
     if st != OK:
         raise LscompError(st, "lscomp_get_launch_shape")
     return int(t.value), int(w.value)
+
+
+def autotune(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, reps: int = 3, stream=None):
+    """Time every launch shape on this volume, apply the fastest per direction.
+
+    Returns the measured table (list of dicts); empty when the shapes came from
+    the tuner's cache for this size class and mode.
+    """
+    import torch
+    _check_volume(vol, dims)
+    work = torch.empty(max_compressed_bytes(dims), dtype=torch.uint8, device=vol.device)
+    dec = torch.empty_like(vol)
+    table = (TuneEntry * 32)()
+    n = ctypes.c_uint32(32)
+    st = _lib.lscomp_autotune_u16(ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(work.data_ptr()), work.numel(),
+                                  ctypes.c_void_p(dec.data_ptr()), _dims(dims), _bins(bins), ctypes.c_float(pooling),
+                                  reps, _stream_handle(stream), table, ctypes.byref(n))
+    if st != OK:
+        raise LscompError(st, "lscomp_autotune_u16")
+    return [{"tile": t.tile, "wg_per_cu": t.wg_per_cu, "compress_ms": t.compress_ms,
+             "decompress_ms": t.decompress_ms} for t in table[: n.value]]
 
 
 def version() -> int:

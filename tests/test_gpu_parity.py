@@ -179,3 +179,21 @@ def test_launch_shapes_give_identical_bytes(gpu, orc, tile, wg_per_cu):
     finally:
         for d in (gpu.COMPRESS, gpu.DECOMPRESS):
             gpu.set_launch_shape(d, 0, 0)
+
+
+def test_autotune_applies_a_shape_and_keeps_bytes(gpu, orc):
+    import torch
+    vol = synth("xpcs", (8, 512, 512), seed=17)
+    d = to_dev(vol)
+    saved = [gpu.get_launch_shape(k) for k in (gpu.COMPRESS, gpu.DECOMPRESS)]
+    try:
+        table = gpu.autotune(d, vol.shape, (1, 1, 1, 1), 1.0, reps=2)
+        assert len(table) == 15 and all(e["compress_ms"] > 0 and e["decompress_ms"] > 0 for e in table)
+        assert gpu.autotune(d, vol.shape, (1, 1, 1, 1), 1.0) == []          # cached
+        tile, w = gpu.get_launch_shape(gpu.COMPRESS)
+        assert tile in (128, 256, 512)
+        check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+    finally:
+        for k, (t, w) in zip((gpu.COMPRESS, gpu.DECOMPRESS), saved):
+            gpu.set_launch_shape(k, t, w)
+    torch.cuda.synchronize()
