@@ -106,6 +106,10 @@ void lscomp_release_all(void);
  * autotuner or by hand. */
 int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu);
 int lscomp_get_launch_shape(int direction, uint32_t *tile, uint32_t *wg_per_cu);
+/* Single-volume decompression: 1 = tile offsets from a header pre-pass
+ * (default), 0 = decoupled look-back inside the decode kernel.  Returns the
+ * previous setting.  Output is identical either way. */
+int lscomp_set_decode_prepass(int enable);
 
 /* One measured launch shape (times in ms, averaged over the repetitions). */
 typedef struct lscomp_tune_entry {

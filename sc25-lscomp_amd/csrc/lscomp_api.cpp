@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <map>
@@ -49,6 +50,12 @@ uint64_t num_blocks(lscomp_dim3 d)
 // per CU (0 = occupancy limit).  Set by lscomp_set_launch_shape / the autotuner.
 std::atomic<uint32_t> g_tile[2] = {{(uint32_t)kDefaultTile}, {(uint32_t)kDefaultTile}};
 std::atomic<uint32_t> g_wgpcu[2] = {{0u}, {0u}};
+// Single-volume decompression computes tile offsets in a pre-pass instead of
+// a look-back (LSCOMP_DECODE_PREPASS=0 in the environment turns it off).
+std::atomic<bool> g_decode_prepass{[] {
+    const char *e = std::getenv("LSCOMP_DECODE_PREPASS");
+    return !(e && e[0] == '0');
+}()};
 
 int hip_fail(hipError_t e, const char *what)
 {
@@ -121,6 +128,7 @@ struct Scratch {
     Ctrl *ctrl = nullptr;          // device, zero-initialised
     uint64_t *status = nullptr;    // device, cap_tiles look-back words
     uint64_t cap_tiles = 0;
+    uint64_t *tile_off = nullptr;  // device, cap_tiles + 1 decode tile offsets
     uint64_t launches = 0;         // epochs used since the status words were cleared
     Ctrl *host = nullptr;          // pinned readback of ctrl
     std::mutex mu;
@@ -169,10 +177,13 @@ int ensure_tiles(Scratch *s, uint64_t tiles)
         int st = hip_fail(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
         if (st) return st;
         (void)hipFree(s->status);
+        (void)hipFree(s->tile_off);
         s->status = nullptr;
+        s->tile_off = nullptr;
         s->cap_tiles = 0;
     }
     int st = hip_fail(hipMalloc((void **)&s->status, cap * sizeof(uint64_t)), "hipMalloc");
+    if (!st) st = hip_fail(hipMalloc((void **)&s->tile_off, (cap + 1) * sizeof(uint64_t)), "hipMalloc");
     if (st) return st;
     st = hip_fail(launch_fill_u64(s->status, cap, 0, s->stream), "fill");
     if (st) return st;
@@ -255,7 +266,13 @@ int launch_item(Scratch *s, Item &it, bool compress)
         if (!it.result) it.result = &s->ctrl->result;
         return hip_fail(launch_compress(it, nullptr, nullptr, cfg, s->ctrl, s->status, s->stream), "compress launch");
     }
-    return hip_fail(launch_decompress(it, nullptr, nullptr, cfg, s->ctrl, s->status, s->stream), "decompress launch");
+    LaunchCfg dcfg = cfg;
+    if (g_decode_prepass.load()) {
+        dcfg.tile_off = s->tile_off;
+        st = count_launch(s);   // the pre-pass is a launch of its own
+        if (st) return st;
+    }
+    return hip_fail(launch_decompress(it, nullptr, nullptr, dcfg, s->ctrl, s->status, s->stream), "decompress launch");
 }
 
 }  // namespace
@@ -366,6 +383,7 @@ void lscomp_release_all(void)
         if (s->device != dev) { ++itr; continue; }
         (void)hipStreamSynchronize(s->stream);
         if (s->status) (void)hipFree(s->status);
+        if (s->tile_off) (void)hipFree(s->tile_off);
         if (s->ctrl) (void)hipFree(s->ctrl);
         if (s->host) (void)hipHostFree(s->host);
         delete s;
@@ -381,6 +399,12 @@ int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu)
     g_tile[direction].store(tile);
     g_wgpcu[direction].store(wg_per_cu);
     return LSCOMP_OK;
+}
+
+int lscomp_set_decode_prepass(int enable)
+{
+    const bool old = g_decode_prepass.exchange(enable != 0);
+    return old ? 1 : 0;
 }
 
 int lscomp_get_launch_shape(int direction, uint32_t *tile, uint32_t *wg_per_cu)

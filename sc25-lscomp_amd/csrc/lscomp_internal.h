@@ -60,6 +60,8 @@ struct LaunchCfg {
     uint32_t total_tiles;
     uint32_t tile;            // blocks per tile: 128, 256 or 512
     const uint32_t *d_total_tiles;   // if set, the tile count is read from device memory
+    uint64_t *tile_off;              // decompress, single item: if set, a pre-pass computes
+                                     // tile offsets (ntiles + 1 words) and the decode needs no look-back
 };
 
 // Kernel launchers (lscomp_kernels.hip).  `items` is a device array of item

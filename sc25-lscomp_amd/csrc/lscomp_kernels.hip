@@ -145,14 +145,20 @@ __device__ __forceinline__ uint32_t tile_scan(uint32_t words, uint32_t tid, uint
     return before + incl - words;
 }
 
+// LDS of the workgroup-wide look-back (NW waves).
+template <int NW>
+struct LbShared {
+    uint64_t incl[NW], empty[NW], sum[NW];
+    uint32_t err;
+};
+
 template <int TILE>
 struct TileShared {
     static constexpr int kWaves = TILE / 64;
     uint32_t pay[TILE * 32];       // payload staging, 4-byte words (128 B per block max)
     uint32_t hdr[TILE / 4];        // header bytes of the tile
     uint32_t wsum[kWaves];
-    uint64_t lb_incl[kWaves], lb_empty[kWaves], lb_sum[kWaves];
-    uint32_t lb_err;
+    LbShared<kWaves> lb;
     uint32_t tile, epoch, ntot;
 };
 
@@ -162,7 +168,7 @@ struct TileShared {
 // flow, barriers inside).
 template <int TILE>
 __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t first, uint32_t epoch,
-                                  TileShared<TILE> &sh, Ctrl *ctrl)
+                                  LbShared<TILE / 64> &sh, Ctrl *ctrl)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint64_t excl = 0;
@@ -178,14 +184,14 @@ __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t firs
             const uint32_t flag = ((uint32_t)(w >> 38) == epoch) ? (uint32_t)(w >> 36) & 3u : 0u;
             const uint64_t incl_m = __ballot(flag == LB_INCL);
             const uint64_t empty_m = __ballot(flag == LB_EMPTY);
-            if (lane == 0) { sh.lb_incl[wave] = incl_m; sh.lb_empty[wave] = empty_m; }
-            if (tid == 0) sh.lb_err = __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) { sh.incl[wave] = incl_m; sh.empty[wave] = empty_m; }
+            if (tid == 0) sh.err = __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
             bool retry = false;
             j = TILE;
 #pragma unroll
             for (int v = 0; v < TILE / 64; v++) {
-                const uint64_t im = sh.lb_incl[v], em = sh.lb_empty[v];
+                const uint64_t im = sh.incl[v], em = sh.empty[v];
                 if (im) {
                     const uint64_t need = ((im & (~im + 1ull)) << 1) - 1ull;
                     retry = (em & need) != 0;
@@ -194,7 +200,7 @@ __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t firs
                 }
                 if (em) { retry = true; break; }
             }
-            const uint32_t err = sh.lb_err;
+            const uint32_t err = sh.err;
             __syncthreads();
             if (!retry) break;
             // Bounded wait: after kSpinLimit polls, or once any tile of the
@@ -207,11 +213,11 @@ __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t firs
             __builtin_amdgcn_s_sleep(1);
         }
         const uint64_t part = wave_sum_u64(tid <= j ? (w & kValMask) : 0ull);
-        if (lane == 0) sh.lb_sum[wave] = part;
+        if (lane == 0) sh.sum[wave] = part;
         __syncthreads();
         uint64_t total = 0;
 #pragma unroll
-        for (int v = 0; v < TILE / 64; v++) total += sh.lb_sum[v];
+        for (int v = 0; v < TILE / 64; v++) total += sh.sum[v];
         __syncthreads();
         excl += total;
         if (j < (uint32_t)TILE) return excl;
@@ -243,8 +249,8 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
     }
 }
 
-template <int TILE>
-__device__ __forceinline__ void next_tile(TileShared<TILE> &sh, Ctrl *ctrl)
+template <class SH>
+__device__ __forceinline__ void next_tile(SH &sh, Ctrl *ctrl)
 {
     if (threadIdx.x == 0)
         sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -305,7 +311,7 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
 
         uint64_t excl = 0;
         if (tl != 0) {
-            excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh, ctrl);
+            excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
         }
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
@@ -334,7 +340,8 @@ template <int TILE, bool kTable>
 __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, const Item *__restrict__ items,
                                                         const uint32_t *__restrict__ tile_items,
                                                           uint32_t total_tiles, const uint32_t *d_total_tiles,
-                                                          Ctrl *ctrl, uint64_t *status)
+                                                          Ctrl *ctrl, uint64_t *status,
+                                                          const uint64_t *__restrict__ tile_off)
 {
     __shared__ TileShared<TILE> sh;
     const uint32_t tid = threadIdx.x;
@@ -355,16 +362,19 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
         const bool valid = b < it.nblocks;
 
         const uint32_t hdr = valid ? (uint32_t)it.s_in[b] : 0u;
+        // with precomputed tile offsets (single item) no look-back is needed
+        uint64_t excl = tile_off ? tile_off[tile] : 0ull;
         const uint32_t words = header_payload_bytes(hdr) >> 2;
         bool bad = (hdr & 0x1fu) > 16u;
 
         uint32_t agg;
         const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
-        if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
-        uint64_t excl = 0;
-        if (tl != 0) {
-            excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh, ctrl);
-            if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
+        if (!tile_off) {
+            if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
+            if (tl != 0) {
+                excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
+                if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
+            }
         }
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
         const bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
@@ -396,6 +406,95 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
     finish_launch(ctrl, epoch);
 }
 
+// ---------------------------------------------------------------------------
+// Decode pre-pass: payload offset of every tile of a single item, from the
+// header bytes alone.  A workgroup of 256 threads takes a chunk of 64 tiles
+// (4 threads per tile, TILE/4 header bytes each), reduces sizes per tile,
+// scans them, and chains chunks with the same look-back.  tile_off[t] is the
+// word offset of tile t's payload after the header section; tile_off[ntiles]
+// is the total.
+// ---------------------------------------------------------------------------
+struct OffShared {
+    uint32_t tagg[64];
+    LbShared<4> lb;
+    uint32_t tile, epoch, ntot;
+    uint32_t excl_lo, excl_hi;
+};
+
+// payload words of the 4 headers packed in one dword
+__device__ __forceinline__ uint32_t words4(uint32_t x)
+{
+    const uint32_t rate = x & 0x1f1f1f1fu;
+    const uint32_t plain = ((~x >> 7) & 0x01010101u) * 0x1fu;   // 0x1f in bytes of plain blocks
+    const uint32_t w = rate + (rate & plain);                     // rate * (plain ? 2 : 1), <= 62 per byte
+    return (w * 0x01010101u) >> 24;                               // byte sum <= 248
+}
+
+template <int TILE>
+__global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl, uint64_t *status,
+                                                     uint64_t *__restrict__ tile_off)
+{
+    __shared__ OffShared sh;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t nchunks = (it.ntiles + 63) / 64;
+    if (tid == 0) sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    next_tile(sh, ctrl);
+    __syncthreads();
+    const uint32_t epoch = sh.epoch;
+    constexpr uint32_t kBytes = TILE / 4;   // header bytes per thread
+
+    for (;;) {
+        const uint32_t chunk = sh.tile;
+        if (chunk >= nchunks) break;
+        const uint32_t j = tid >> 2, q = tid & 3u;
+        const uint32_t tile = chunk * 64 + j;
+        const uint64_t first = (uint64_t)tile * TILE + q * kBytes;   // first header byte of this thread
+        uint32_t sum = 0;
+        if (first < it.nblocks) {
+            const uint32_t *src = (const uint32_t *)(it.s_in + first);
+            const uint32_t nvalid = (uint32_t)((it.nblocks - first) < kBytes ? (it.nblocks - first) : kBytes);
+#pragma unroll
+            for (uint32_t k = 0; k < kBytes / 4; k++) {
+                if (4 * k >= nvalid) break;   // never read past the header section
+                uint32_t x = src[k];
+                const uint32_t keep = nvalid - 4 * k;
+                if (keep < 4) x &= 0xffffffffu >> (32 - 8 * keep);   // clip bytes past the last block
+                sum += words4(x);
+            }
+        }
+        sum += __shfl_xor(sum, 1, 64);
+        sum += __shfl_xor(sum, 2, 64);
+        if (q == 0) sh.tagg[j] = sum;
+        __syncthreads();
+        uint32_t incl = 0, agg = 0;
+        if (tid < 64) {
+            incl = wave_incl_scan(sh.tagg[lane], lane);
+            agg = __shfl(incl, 63, 64);
+        }
+        // chunk aggregate published by thread 0, look-back over chunks
+        if (tid == 0) {
+            sh.excl_lo = agg;
+            st_relaxed(status + chunk, lb_word(epoch, chunk == 0 ? LB_INCL : LB_AGG, agg));
+        }
+        __syncthreads();
+        const uint32_t cagg = sh.excl_lo;
+        uint64_t excl = 0;
+        if (chunk != 0) {
+            excl = lookback_tile<256>(status, chunk, 0, epoch, sh.lb, ctrl);
+            if (tid == 0) st_relaxed(status + chunk, lb_word(epoch, LB_INCL, excl + cagg));
+        }
+        if (tid < 64) {
+            const uint32_t t = chunk * 64 + lane;
+            if (t < it.ntiles) tile_off[t] = excl + incl - sh.tagg[lane];
+            if (t == it.ntiles - 1) tile_off[it.ntiles] = excl + incl;
+        }
+        __syncthreads();
+        next_tile(sh, ctrl);
+        __syncthreads();
+    }
+    finish_launch(ctrl, epoch);
+}
+
 __global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t value)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -418,10 +517,18 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
     } else {
         if (items)
             hipLaunchKernelGGL((decompress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
-                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status, (const uint64_t *)nullptr);
         else
+        {
+            if (cfg.tile_off) {
+                const uint32_t nchunks = (single.ntiles + 63) / 64;
+                const uint32_t g = nchunks < cfg.grid ? nchunks : cfg.grid;
+                hipLaunchKernelGGL((offsets_kernel<TILE>), dim3(g), dim3(256), 0, stream, single, ctrl, status,
+                                   cfg.tile_off);
+            }
             hipLaunchKernelGGL((decompress_kernel<TILE, false>), grid, block, 0, stream, single, items, tile_items,
-                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status, (const uint64_t *)cfg.tile_off);
+        }
     }
     return hipGetLastError();
 }

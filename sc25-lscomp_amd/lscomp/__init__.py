@@ -62,7 +62,7 @@ EXPORTS = (
     "lscomp_plan_launch_graph", "lscomp_plan_device_sizes", "lscomp_plan_sizes", "lscomp_plan_status",
     "lscomp_plan_destroy", "lscomp_server_create", "lscomp_server_submit_u16", "lscomp_server_poll",
     "lscomp_server_release", "lscomp_server_drain", "lscomp_server_replay_u16", "lscomp_server_destroy",
-    "lscomp_autotune_u16",
+    "lscomp_autotune_u16", "lscomp_set_decode_prepass",
 )
 
 
@@ -143,6 +143,8 @@ def _load():
                                              ctypes.c_uint32, ctypes.POINTER(Completion)]
     lib.lscomp_server_destroy.restype = ctypes.c_int
     lib.lscomp_server_destroy.argtypes = [vp]
+    lib.lscomp_set_decode_prepass.restype = ctypes.c_int
+    lib.lscomp_set_decode_prepass.argtypes = [ctypes.c_int]
     lib.lscomp_autotune_u16.restype = ctypes.c_int
     lib.lscomp_autotune_u16.argtypes = [vp, vp, sz, vp, Dim3, Bins4, ctypes.c_float, ctypes.c_int, vp,
                                         ctypes.POINTER(TuneEntry), ctypes.POINTER(ctypes.c_uint32)]
@@ -303,6 +305,11 @@ def autotune(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, reps: int = 3,
         raise LscompError(st, "lscomp_autotune_u16")
     return [{"tile": t.tile, "wg_per_cu": t.wg_per_cu, "compress_ms": t.compress_ms,
              "decompress_ms": t.decompress_ms} for t in table[: n.value]]
+
+
+def set_decode_prepass(enable: bool) -> bool:
+    """Single-volume decode: tile offsets by pre-pass (True, default) or look-back; returns the old setting."""
+    return bool(_lib.lscomp_set_decode_prepass(1 if enable else 0))
 
 
 def version() -> int:

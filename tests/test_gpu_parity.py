@@ -197,3 +197,17 @@ def test_autotune_applies_a_shape_and_keeps_bytes(gpu, orc):
         for k, (t, w) in zip((gpu.COMPRESS, gpu.DECOMPRESS), saved):
             gpu.set_launch_shape(k, t, w)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("prepass", [False, True])
+def test_decode_with_and_without_prepass(gpu, orc, prepass):
+    """Decode tile offsets from the header pre-pass or from the in-kernel look-back: same volume."""
+    old = gpu.set_decode_prepass(prepass)
+    try:
+        for shape, kind in (((3, 300, 1558), "cssi"), ((1, 8, 9), "uniform"), ((64, 128, 128), "xpcs"),
+                            ((2, 2048, 2056), "mixed")):
+            vol = synth(kind, shape, seed=31)
+            check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+            check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
+    finally:
+        gpu.set_decode_prepass(old)
