@@ -28,7 +28,8 @@ LSC_HD u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
 LSC_HD uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 // v_perm_b32: result byte i = byte sel.byte[i] of the 8-byte value {hi:lo}
-// (selector 0..3 picks from lo, 4..7 from hi).  Only selectors 0..7 are used.
+// (selector 0..3 picks from lo, 4..7 from hi, 0x0c gives 0x00).  Only these
+// selectors are used.
 LSC_HD uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -37,8 +38,9 @@ LSC_HD uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
     uint64_t v = ((uint64_t)hi << 32) | lo;
     uint32_t r = 0;
     for (int i = 0; i < 4; i++) {
-        uint32_t s = (sel >> (8 * i)) & 7u;
-        r |= (uint32_t)((v >> (8 * s)) & 0xffu) << (8 * i);
+        const uint32_t s = (sel >> (8 * i)) & 0xffu;
+        if (s == 0x0c) continue;   // constant zero byte
+        r |= (uint32_t)((v >> (8 * (s & 7u))) & 0xffu) << (8 * i);
     }
     return r;
 #endif
@@ -149,21 +151,20 @@ LSC_HD uint32_t header_payload_bytes(uint32_t hdr)
 // (8 bytes, byte r = row r, bit 7-k = bit i of q[r][k]) is then
 //   OR_k rotr(C[k], i + k - 7) & (0x01010101 << (7 - k))   per 4-row half.
 // ---------------------------------------------------------------------------
-LSC_HD void columns_from_rows(const uint32_t D[32], uint32_t CL[16], uint32_t CH[16])
+// Column words of the low bytes (hi = false) or the high bytes (hi = true).
+LSC_HD void columns_from_rows(const uint32_t D[32], uint32_t C[16], bool hi)
 {
+    const uint32_t s01 = hi ? 0x07030501u : 0x06020400u;
 #pragma unroll
     for (int h = 0; h < 2; h++)           // row half: rows 4h..4h+3
 #pragma unroll
         for (int c = 0; c < 4; c++) {     // column pair (2c, 2c+1)
             const uint32_t d0 = D[4 * (4 * h + 0) + c], d1 = D[4 * (4 * h + 1) + c];
             const uint32_t d2 = D[4 * (4 * h + 2) + c], d3 = D[4 * (4 * h + 3) + c];
-            // low bytes: (row0 k, row1 k, row0 k+1, row1 k+1)
-            uint32_t l01 = perm(d1, d0, 0x06020400u), l23 = perm(d3, d2, 0x06020400u);
-            uint32_t h01 = perm(d1, d0, 0x07030501u), h23 = perm(d3, d2, 0x07030501u);
-            CL[8 * h + 2 * c] = perm(l23, l01, 0x05040100u);
-            CL[8 * h + 2 * c + 1] = perm(l23, l01, 0x07060302u);
-            CH[8 * h + 2 * c] = perm(h23, h01, 0x05040100u);
-            CH[8 * h + 2 * c + 1] = perm(h23, h01, 0x07060302u);
+            // (row0 k, row1 k, row0 k+1, row1 k+1) of the selected byte
+            const uint32_t x01 = perm(d1, d0, s01), x23 = perm(d3, d2, s01);
+            C[8 * h + 2 * c] = perm(x23, x01, 0x05040100u);
+            C[8 * h + 2 * c + 1] = perm(x23, x01, 0x07060302u);
         }
 }
 
@@ -177,6 +178,24 @@ LSC_HD void rows_from_columns(const uint32_t CL[16], const uint32_t CH[16], uint
             const uint32_t he = CH[8 * h + 2 * c], ho = CH[8 * h + 2 * c + 1];
             uint32_t e01 = perm(he, le, 0x05010400u), e23 = perm(he, le, 0x07030602u);
             uint32_t o01 = perm(ho, lo, 0x05010400u), o23 = perm(ho, lo, 0x07030602u);
+            D[4 * (4 * h + 0) + c] = perm(o01, e01, 0x05040100u);
+            D[4 * (4 * h + 1) + c] = perm(o01, e01, 0x07060302u);
+            D[4 * (4 * h + 2) + c] = perm(o23, e23, 0x05040100u);
+            D[4 * (4 * h + 3) + c] = perm(o23, e23, 0x07060302u);
+        }
+}
+
+// rows_from_columns when every high byte is zero (rate <= 8)
+LSC_HD void rows_from_low_columns(const uint32_t CL[16], uint32_t D[32])
+{
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t le = CL[8 * h + 2 * c], lo = CL[8 * h + 2 * c + 1];
+            // selector 0x0c yields a zero byte
+            uint32_t e01 = perm(0u, le, 0x0c010c00u), e23 = perm(0u, le, 0x0c030c02u);
+            uint32_t o01 = perm(0u, lo, 0x0c010c00u), o23 = perm(0u, lo, 0x0c030c02u);
             D[4 * (4 * h + 0) + c] = perm(o01, e01, 0x05040100u);
             D[4 * (4 * h + 1) + c] = perm(o01, e01, 0x07060302u);
             D[4 * (4 * h + 2) + c] = perm(o23, e23, 0x05040100u);
@@ -222,16 +241,16 @@ LSC_HD void quantize_pooled(const uint32_t D[32], float f, bool identity, uint32
     }
 }
 
-// Pooled column words: CP[u] byte m = low byte of Q[8m+u], CPH the high byte.
-LSC_HD void pooled_columns(const uint32_t Q[32], uint32_t CP[8], uint32_t CPH[8])
+// Pooled column words: C[u] byte m = low (hi = false) or high byte of Q[8m+u].
+LSC_HD void pooled_columns(const uint32_t Q[32], uint32_t C[8], bool hi)
 {
+    const uint32_t sel = hi ? 0x07060302u : 0x05040100u;
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-        uint32_t a = Q[u], b = Q[8 + u], c = Q[16 + u], d = Q[24 + u];
-        uint32_t ab = perm(b, a, 0x05010400u);   // (a.lo, b.lo, a.hi, b.hi)
-        uint32_t cd = perm(d, c, 0x05010400u);   // (c.lo, d.lo, c.hi, d.hi)
-        CP[u] = perm(cd, ab, 0x05040100u);       // (a.lo, b.lo, c.lo, d.lo)
-        CPH[u] = perm(cd, ab, 0x07060302u);      // (a.hi, b.hi, c.hi, d.hi)
+        const uint32_t a = Q[u], b = Q[8 + u], c = Q[16 + u], d = Q[24 + u];
+        const uint32_t ab = perm(b, a, 0x05010400u);   // (a.lo, b.lo, a.hi, b.hi)
+        const uint32_t cd = perm(d, c, 0x05010400u);   // (c.lo, d.lo, c.hi, d.hi)
+        C[u] = perm(cd, ab, sel);                      // (a, b, c, d) low or high bytes
     }
 }
 
@@ -246,25 +265,31 @@ LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint
     const bool identity = p.b[bc] == 1u;
     const float f = bc == 0 ? p.f[0] : bc == 1 ? p.f[1] : bc == 2 ? p.f[2] : p.f[3];
     if (hdr & 0x80u) {
-        uint32_t Q[32], CP[8], CPH[8];
+        uint32_t Q[32], C[8];
         quantize_pooled(D, f, identity, Q);
-        pooled_columns(Q, CP, CPH);
+        pooled_columns(Q, C, false);
         const uint32_t nlo = rate < 8 ? rate : 8u;
-        for (uint32_t i = 0; i < nlo; i++) out[i] = plane_from_columns(CP, i);
-        for (uint32_t i = 8; i < rate; i++) out[i] = plane_from_columns(CPH, i - 8);
+        for (uint32_t i = 0; i < nlo; i++) out[i] = plane_from_columns(C, i);
+        if (rate > 8) {
+            pooled_columns(Q, C, true);
+            for (uint32_t i = 8; i < rate; i++) out[i] = plane_from_columns(C, i - 8);
+        }
         return rate;
     }
     if (!identity) quantize_plain(D, f);
-    uint32_t CL[16], CH[16];
-    columns_from_rows(D, CL, CH);
+    uint32_t C[16];
+    columns_from_rows(D, C, false);
     const uint32_t nlo = rate < 8 ? rate : 8u;
     for (uint32_t i = 0; i < nlo; i++) {
-        out[2 * i] = plane_from_columns(CL, i);
-        out[2 * i + 1] = plane_from_columns(CL + 8, i);
+        out[2 * i] = plane_from_columns(C, i);
+        out[2 * i + 1] = plane_from_columns(C + 8, i);
     }
-    for (uint32_t i = 8; i < rate; i++) {
-        out[2 * i] = plane_from_columns(CH, i - 8);
-        out[2 * i + 1] = plane_from_columns(CH + 8, i - 8);
+    if (rate > 8) {
+        columns_from_rows(D, C, true);
+        for (uint32_t i = 8; i < rate; i++) {
+            out[2 * i] = plane_from_columns(C, i - 8);
+            out[2 * i + 1] = plane_from_columns(C + 8, i - 8);
+        }
     }
     return 2 * rate;
 }
@@ -306,7 +331,8 @@ LSC_HD void decode_block(const uint32_t *in, uint32_t hdr, const uint32_t bins[4
         columns_add_plane(CH, in[2 * i], i - 8);
         columns_add_plane(CH + 8, in[2 * i + 1], i - 8);
     }
-    rows_from_columns(CL, CH, D);
+    if (rate > 8) rows_from_columns(CL, CH, D);
+    else rows_from_low_columns(CL, D);
     if (bin != 1u) {
         const u16x2 bb = {(unsigned short)bin, (unsigned short)bin};
 #pragma unroll

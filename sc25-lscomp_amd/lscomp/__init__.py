@@ -17,7 +17,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblscomp.so")
+# LSCOMP_LIBRARY selects another build of the same library (e.g. a variant
+# compiled with different flags); default: the in-tree build.
+LIB_PATH = os.environ.get("LSCOMP_LIBRARY") or os.path.join(_HERE, "liblscomp.so")
 
 OK = 0
 ERR_INVALID_ARG = 1
