@@ -1,0 +1,47 @@
+"""The lsCOMP_xpcs drop-in CLI, driven like the reference's AE scripts
+(1-reproducing-main-3-1.py:8-18): same flags, same stdout lines, and -o writes
+the decompressed volume (exact for lossless, oracle-equal for lossy)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from lsc_helpers import synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "sc25-lscomp_amd", "cli", "lsCOMP_xpcs")
+
+
+def _parse(stdout):
+    comp = dec = ratio = None
+    for line in stdout.splitlines():
+        if "compression   end-to-end speed" in line:
+            comp = float(line.strip().split()[-2])
+        elif "decompression end-to-end speed" in line:
+            dec = float(line.strip().split()[-2])
+        elif "compression ratio" in line:
+            ratio = float(line.strip().split()[-1])
+    return comp, dec, ratio
+
+
+@pytest.mark.parametrize("bins,p", [((1, 1, 1, 1), 1.0), ((3, 5, 10, 15), 0.5)])
+def test_cli_roundtrip(orc, tmp_path, bins, p):
+    assert os.path.exists(CLI), "CLI not built"
+    vol = synth("cssi", (4, 181, 155), seed=8)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    vol.tofile(src)
+    cmd = [CLI, "-i", str(src), "-d", *map(str, vol.shape), "-b", *map(str, bins), "-p", str(p), "-o", str(dst)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    comp, dec, ratio = _parse(r.stdout)
+    assert comp > 0 and dec > 0
+    st, ref, refn = orc.compress(vol, bins, p)
+    assert abs(ratio - vol.nbytes / refn) < 1e-4
+    out = np.fromfile(dst, np.uint16).reshape(vol.shape)
+    _, refdec = orc.decompress(ref, vol.shape, bins)
+    assert np.array_equal(out, refdec)
+    if bins == (1, 1, 1, 1):
+        assert np.array_equal(out, vol)
