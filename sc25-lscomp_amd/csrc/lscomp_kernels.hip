@@ -314,6 +314,7 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
             excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
         }
+        __syncthreads();   // every lane's payload is in LDS before the copy below
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
         // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0)
         for (uint32_t w = tid; w < (uint32_t)TILE / 4; w += TILE) {
