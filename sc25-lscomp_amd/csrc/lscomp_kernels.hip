@@ -1,16 +1,19 @@
 // lscomp_kernels.hip -- gfx950 kernels of the lsCOMP uint16 codec.
 //
-// One workgroup = one tile of kTile consecutive 8x8 blocks of an item, one
-// block per lane.  Compression (replaces cuLSZ_compression_kernel_uint16_bsize64,
+// One workgroup = one tile of TILE (128/256/512) consecutive 8x8 blocks of an
+// item, one block per lane; persistent workgroups loop over tiles.
+// Compression (replaces cuLSZ_compression_kernel_uint16_bsize64,
 // lsCOMP/src/cuLSZ_kernel.cu:827-1283):
-//   1. fetch a dynamic tile id (forward progress does not depend on dispatch order);
+//   1. dynamic tile ids (forward progress does not depend on dispatch order),
+//      the next id prefetched while the current tile runs;
 //   2. each lane loads its block once (8 row loads, coalesced across lanes:
 //      lanes are consecutive blocks along z), computes the header byte;
 //   3. workgroup scan of payload sizes, tile aggregate published at once;
 //   4. payload packed into LDS while predecessors resolve;
-//   5. wave-wide decoupled look-back over 64 predecessors per probe, one
-//      64-bit word per tile (epoch | flag | value) read and written with
-//      relaxed agent-scope atomics, so no fences are needed on the multi-XCD part;
+//   5. decoupled look-back with a window of TILE predecessors per probe (one
+//      per thread), one 64-bit word per tile (epoch | flag | value) read and
+//      written with relaxed agent-scope atomics, so no fences are needed on
+//      the multi-XCD part;
 //   6. headers and payload leave LDS as contiguous coalesced dword stores.
 // Decompression (replaces cuLSZ_decompression_kernel_uint16_bsize64,
 // cuLSZ_kernel.cu:1286-1646) runs the same scan over header sizes, stages the
@@ -159,7 +162,7 @@ struct TileShared {
     uint32_t hdr[TILE / 4];        // header bytes of the tile
     uint32_t wsum[kWaves];
     LbShared<kWaves> lb;
-    uint32_t tile, epoch, ntot;
+    uint32_t tile, next, epoch, ntot;
 };
 
 // Exclusive prefix (in words) of tile `tile` over the tiles [first, tile) of
@@ -249,11 +252,28 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
     }
 }
 
+// Dynamic tile ids.  first_tile() takes the workgroup's first id; at the top
+// of each iteration prefetch_tile() already takes the next one, so the atomic's
+// latency overlaps the current tile; advance_tile() (between barriers) makes
+// the prefetched id current.
 template <class SH>
-__device__ __forceinline__ void next_tile(SH &sh, Ctrl *ctrl)
+__device__ __forceinline__ void first_tile(SH &sh, Ctrl *ctrl)
 {
     if (threadIdx.x == 0)
         sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class SH>
+__device__ __forceinline__ void prefetch_tile(SH &sh, Ctrl *ctrl)
+{
+    if (threadIdx.x == 0)
+        sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class SH>
+__device__ __forceinline__ void advance_tile(SH &sh)
+{
+    if (threadIdx.x == 0) sh.tile = sh.next;
 }
 
 // Item of a global tile: the single item, or items[tile_item[tile]].
@@ -280,13 +300,14 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
     }
-    next_tile(sh, ctrl);
+    first_tile(sh, ctrl);
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
+        prefetch_tile(sh, ctrl);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
         const uint32_t b = tl * (uint32_t)TILE + tid;
@@ -328,7 +349,7 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
         if (tid == 0 && tl == it.ntiles - 1 && it.result) *it.result = g0 + 4ull * agg;
 
         __syncthreads();
-        next_tile(sh, ctrl);
+        advance_tile(sh);
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
@@ -350,13 +371,14 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
     }
-    next_tile(sh, ctrl);
+    first_tile(sh, ctrl);
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
+        prefetch_tile(sh, ctrl);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
         const uint32_t b = tl * (uint32_t)TILE + tid;
@@ -401,7 +423,7 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
                                            __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        next_tile(sh, ctrl);
+        advance_tile(sh);
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
@@ -418,7 +440,7 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
 struct OffShared {
     uint32_t tagg[64];
     LbShared<4> lb;
-    uint32_t tile, epoch, ntot;
+    uint32_t tile, next, epoch, ntot;
     uint32_t excl_lo, excl_hi;
 };
 
@@ -439,7 +461,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t nchunks = (it.ntiles + 63) / 64;
     if (tid == 0) sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    next_tile(sh, ctrl);
+    first_tile(sh, ctrl);
     __syncthreads();
     const uint32_t epoch = sh.epoch;
     constexpr uint32_t kBytes = TILE / 4;   // header bytes per thread
@@ -447,6 +469,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
     for (;;) {
         const uint32_t chunk = sh.tile;
         if (chunk >= nchunks) break;
+        prefetch_tile(sh, ctrl);
         const uint32_t j = tid >> 2, q = tid & 3u;
         const uint32_t tile = chunk * 64 + j;
         const uint64_t first = (uint64_t)tile * TILE + q * kBytes;   // first header byte of this thread
@@ -490,7 +513,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
             if (t == it.ntiles - 1) tile_off[it.ntiles] = excl + incl;
         }
         __syncthreads();
-        next_tile(sh, ctrl);
+        advance_tile(sh);
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
