@@ -304,6 +304,20 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
+    // Software pipeline: D holds the block of the current tile; the loads of
+    // the next tile are issued as soon as the current one is packed, so they
+    // are in flight during the look-back and the stores.
+    uint32_t D[32];
+    if (sh.tile < ntot) {
+        const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
+        const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
+        if (b < it.nblocks) {
+            uint32_t bx, by, bz;
+            block_coords(it, b, bx, by, bz);
+            load_block(it, bx, by, bz, D);
+        }
+    }
+
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
@@ -313,22 +327,29 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
         const uint32_t b = tl * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
 
-        uint32_t D[32];
         uint32_t hdr = 0, words = 0;
         if (valid) {
-            uint32_t bx, by, bz;
-            block_coords(it, b, bx, by, bz);
-            load_block(it, bx, by, bz, D);
             hdr = block_header(D, it.dec);
             words = header_payload_bytes(hdr) >> 2;
         }
         ((uint8_t *)sh.hdr)[tid] = (uint8_t)hdr;
 
         uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);   // barrier: sh.next is visible
         if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
 
         if (words) encode_block(D, hdr, it.dec, &sh.pay[off]);
+
+        const uint32_t ntile = sh.next;
+        if (ntile < ntot) {
+            const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
+            const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
+            if (nb < nit.nblocks) {
+                uint32_t bx, by, bz;
+                block_coords(nit, nb, bx, by, bz);
+                load_block(nit, bx, by, bz, D);
+            }
+        }
 
         uint64_t excl = 0;
         if (tl != 0) {
