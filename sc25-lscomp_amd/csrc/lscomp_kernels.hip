@@ -381,7 +381,7 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
 // ---------------------------------------------------------------------------
 template <int TILE, bool kTable>
 __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, const Item *__restrict__ items,
-                                                        const uint32_t *__restrict__ tile_items,
+                                                          const uint32_t *__restrict__ tile_items,
                                                           uint32_t total_tiles, const uint32_t *d_total_tiles,
                                                           Ctrl *ctrl, uint64_t *status,
                                                           const uint64_t *__restrict__ tile_off)
@@ -396,6 +396,17 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
+    // Pipeline: the header byte and (with precomputed offsets) the payload
+    // offset of the next tile are loaded while the current tile decodes.
+    uint32_t hdr = 0;
+    uint64_t excl_next = 0;
+    if (sh.tile < ntot) {
+        const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
+        const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
+        hdr = b < it.nblocks ? (uint32_t)it.s_in[b] : 0u;
+        if (tile_off) excl_next = tile_off[sh.tile];
+    }
+
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
@@ -405,16 +416,15 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
         const uint32_t b = tl * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
 
-        const uint32_t hdr = valid ? (uint32_t)it.s_in[b] : 0u;
-        // with precomputed tile offsets (single item) no look-back is needed
-        uint64_t excl = tile_off ? tile_off[tile] : 0ull;
+        uint64_t excl = excl_next;   // with precomputed tile offsets no look-back is needed
         const uint32_t words = header_payload_bytes(hdr) >> 2;
         bool bad = (hdr & 0x1fu) > 16u;
 
         uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);   // barrier: sh.next is visible
         if (!tile_off) {
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
+            excl = 0;
             if (tl != 0) {
                 excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
                 if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
@@ -423,16 +433,38 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
         const bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
         if (fits) {
+            // stage the tile's payload: 8 loads in flight per lane before the LDS writes
             const uint32_t *src = (const uint32_t *)(it.s_in + g0);
-            for (uint32_t w = tid; w < agg; w += TILE) sh.pay[w] = src[w];
+            for (uint32_t base = 0; base < agg; base += 8u * TILE) {
+                uint32_t v[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t w = base + (uint32_t)k * TILE + tid;
+                    v[k] = w < agg ? src[w] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t w = base + (uint32_t)k * TILE + tid;
+                    if (w < agg) sh.pay[w] = v[k];
+                }
+            }
+        }
+        // next tile's header byte and offset
+        const uint32_t hdr_cur = hdr;
+        const uint32_t ntile = sh.next;
+        if (ntile < ntot) {
+            const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
+            const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
+            hdr = nb < nit.nblocks ? (uint32_t)nit.s_in[nb] : 0u;
+            if (tile_off) excl_next = tile_off[ntile];
         }
         __syncthreads();
 
         if (valid) {
             uint32_t D[32];
             bad = bad || !fits;
-            if (!bad && (hdr & 0x1fu)) {
-                decode_block(&sh.pay[off], hdr, it.dec.b, D);
+            if (!bad && (hdr_cur & 0x1fu)) {
+                decode_block(&sh.pay[off], hdr_cur, it.dec.b, D);
             } else {
 #pragma unroll
                 for (int i = 0; i < 32; i++) D[i] = 0;
