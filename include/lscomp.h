@@ -219,6 +219,24 @@ int lscomp_server_replay_u16(lscomp_server srv, const lscomp_request_u16 *reqs, 
                              uint32_t n, lscomp_completion *out);
 int lscomp_server_destroy(lscomp_server srv);
 
+/* ---- Error metrics ------------------------------------------------------
+ * Quality of a lossy round trip, computed on the GPU in one pass over both
+ * volumes; mse, range, psnr, norm_err, norm_err_norm and nrmse are the six
+ * results of the reference's computePSNR (lsCOMP/src/cuLSZ_utility.cu:887-946),
+ * with exact integer sums for uint16 data (valid while the sums fit 64 bits). */
+typedef struct lscomp_error_stats {
+    uint64_t n;
+    uint32_t max_abs_err;          /* max |dec - orig| */
+    uint32_t min_orig, max_orig;
+    uint32_t reserved;
+    uint64_t sum_sq_err, sum_sq_dec;
+    double max_rel_err;            /* max |dec - orig| / orig over orig != 0 */
+    double mse, range, psnr, norm_err, norm_err_norm, nrmse;
+} lscomp_error_stats;
+
+int lscomp_error_stats_u16(const uint16_t *d_orig, const uint16_t *d_dec, size_t n, lscomp_error_stats *out,
+                           hipStream_t stream);
+
 const char *lscomp_status_string(int status);
 
 /* Library version, major*10000 + minor*100 + patch. */

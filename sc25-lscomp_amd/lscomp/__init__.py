@@ -64,8 +64,17 @@ EXPORTS = (
     "lscomp_plan_launch_graph", "lscomp_plan_device_sizes", "lscomp_plan_sizes", "lscomp_plan_status",
     "lscomp_plan_destroy", "lscomp_server_create", "lscomp_server_submit_u16", "lscomp_server_poll",
     "lscomp_server_release", "lscomp_server_drain", "lscomp_server_replay_u16", "lscomp_server_destroy",
-    "lscomp_autotune_u16", "lscomp_set_decode_prepass",
+    "lscomp_autotune_u16", "lscomp_set_decode_prepass", "lscomp_error_stats_u16",
 )
+
+
+class ErrorStats(ctypes.Structure):
+    """lscomp_error_stats."""
+    _fields_ = [("n", ctypes.c_uint64), ("max_abs_err", ctypes.c_uint32), ("min_orig", ctypes.c_uint32),
+                ("max_orig", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("sum_sq_err", ctypes.c_uint64),
+                ("sum_sq_dec", ctypes.c_uint64), ("max_rel_err", ctypes.c_double), ("mse", ctypes.c_double),
+                ("range", ctypes.c_double), ("psnr", ctypes.c_double), ("norm_err", ctypes.c_double),
+                ("norm_err_norm", ctypes.c_double), ("nrmse", ctypes.c_double)]
 
 
 class TuneEntry(ctypes.Structure):
@@ -145,6 +154,8 @@ def _load():
                                              ctypes.c_uint32, ctypes.POINTER(Completion)]
     lib.lscomp_server_destroy.restype = ctypes.c_int
     lib.lscomp_server_destroy.argtypes = [vp]
+    lib.lscomp_error_stats_u16.restype = ctypes.c_int
+    lib.lscomp_error_stats_u16.argtypes = [vp, vp, sz, ctypes.POINTER(ErrorStats), vp]
     lib.lscomp_set_decode_prepass.restype = ctypes.c_int
     lib.lscomp_set_decode_prepass.argtypes = [ctypes.c_int]
     lib.lscomp_autotune_u16.restype = ctypes.c_int
@@ -307,6 +318,18 @@ def autotune(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, reps: int = 3,
         raise LscompError(st, "lscomp_autotune_u16")
     return [{"tile": t.tile, "wg_per_cu": t.wg_per_cu, "compress_ms": t.compress_ms,
              "decompress_ms": t.decompress_ms} for t in table[: n.value]]
+
+
+def error_stats(orig, dec, stream=None) -> dict:
+    """GPU error metrics between two device volumes of 2-byte elements (uint16 bits)."""
+    if orig.numel() != dec.numel() or orig.element_size() != 2 or dec.element_size() != 2:
+        raise ValueError("orig and dec must be 2-byte tensors of equal size")
+    out = ErrorStats()
+    st = _lib.lscomp_error_stats_u16(ctypes.c_void_p(orig.data_ptr()), ctypes.c_void_p(dec.data_ptr()),
+                                     orig.numel(), ctypes.byref(out), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_error_stats_u16")
+    return {name: getattr(out, name) for name, _ in ErrorStats._fields_ if name != "reserved"}
 
 
 def set_decode_prepass(enable: bool) -> bool:

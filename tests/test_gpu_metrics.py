@@ -1,0 +1,44 @@
+"""GPU error metrics against a float64 numpy restatement of the reference's
+computePSNR definitions (lsCOMP/src/cuLSZ_utility.cu:887-946)."""
+import numpy as np
+import pytest
+
+from lsc_helpers import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,offset", [(1, 0), (7, 0), (4096, 0), (1000003, 0), (1000003, 1)])
+def test_metrics_vs_numpy(gpu, n, offset):
+    import torch
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 65536, n + 1).astype(np.uint16)[offset:offset + n]
+    d = np.clip(a.astype(np.int64) + rng.integers(-40, 41, n), 0, 65535).astype(np.uint16)
+    ta = torch.from_numpy(np.concatenate([np.zeros(offset, np.uint16), a]).view(np.int16)).cuda()[offset:]
+    td = torch.from_numpy(np.concatenate([np.zeros(offset, np.uint16), d]).view(np.int16)).cuda()[offset:]
+    s = gpu.error_stats(ta, td)
+    a64, d64 = a.astype(np.float64), d.astype(np.float64)
+    err = np.abs(d64 - a64)
+    assert s["n"] == n and s["max_abs_err"] == int(err.max())
+    assert s["sum_sq_err"] == int((err.astype(np.int64) ** 2).sum())
+    assert s["min_orig"] == int(a.min()) and s["max_orig"] == int(a.max())
+    mse = (err ** 2).mean()
+    rng_ = a64.max() - a64.min()
+    if mse > 0 and rng_ > 0:
+        assert s["psnr"] == pytest.approx(20 * np.log10(rng_) - 10 * np.log10(mse), rel=1e-12)
+        assert s["nrmse"] == pytest.approx(np.sqrt(mse) / rng_, rel=1e-12)
+    nz = a64 != 0
+    if nz.any():
+        assert s["max_rel_err"] == pytest.approx((err[nz] / a64[nz]).max(), rel=1e-6)
+
+
+def test_metrics_of_a_lossy_round_trip(gpu):
+    import torch
+    vol = synth("cssi", (4, 256, 520), seed=3)
+    d = torch.from_numpy(vol.view(np.int16)).cuda()
+    out, n = gpu.compress(d, vol.shape, (1, 1, 1, 1), 1.0)
+    s = gpu.error_stats(d, gpu.decompress(out, n, vol.shape))
+    assert s["max_abs_err"] == 0 and s["sum_sq_err"] == 0
+    out, n = gpu.compress(d, vol.shape, (3, 5, 10, 15), 1.0)        # no pooling: error < bin
+    s = gpu.error_stats(d, gpu.decompress(out, n, vol.shape, (3, 5, 10, 15)))
+    assert 0 < s["max_abs_err"] <= 14 and np.isfinite(s["psnr"])
