@@ -276,7 +276,7 @@ int lscomp_server_create(uint32_t slots, uint32_t max_items, uint64_t max_blocks
 
 int lscomp_server_submit_u16(lscomp_server s, const lscomp_request_u16 *req, uint64_t *id)
 {
-    if (!s || !req || !req->d_frame) return LSCOMP_ERR_INVALID_ARG;
+    if (!s || !req || !req->d_frame || ((uintptr_t)req->d_frame & 1)) return LSCOMP_ERR_INVALID_ARG;
     lsc::Item probe;
     int st = lsc::make_item(req->dims, req->bins, req->pooling_th, probe, 128);
     if (st) return st;
