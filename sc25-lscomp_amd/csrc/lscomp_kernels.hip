@@ -66,20 +66,15 @@ __device__ __forceinline__ void load_block(const Item &it, uint32_t bx, uint32_t
     const uint32_t y0 = by * 8, z0 = bz * 8;
     const uint16_t *p = it.vol_in + ((uint64_t)bx * it.y + y0) * it.z + z0;
     const bool full = (y0 + 8 <= it.y) && (z0 + 8 <= it.z);
-    if (full && it.align == ALIGN16) {
+    if (full && it.align != ALIGN2) {
+        // 16-byte row loads; global_load_dwordx4 needs only dword alignment,
+        // so rows of 4-byte-aligned volumes (z even) load whole as well
         uint4 t[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) t[r] = *(const uint4 *)(p + (uint64_t)r * it.z);
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             D[4 * r] = t[r].x; D[4 * r + 1] = t[r].y; D[4 * r + 2] = t[r].z; D[4 * r + 3] = t[r].w;
-        }
-    } else if (full && it.align == ALIGN4) {
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const uint32_t *q = (const uint32_t *)(p + (uint64_t)r * it.z);
-#pragma unroll
-            for (int c = 0; c < 4; c++) D[4 * r + c] = q[c];
         }
     } else {
 #pragma unroll
@@ -102,17 +97,11 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
     const uint32_t y0 = by * 8, z0 = bz * 8;
     uint16_t *p = it.vol_out + ((uint64_t)bx * it.y + y0) * it.z + z0;
     const bool full = (y0 + 8 <= it.y) && (z0 + 8 <= it.z);
-    if (full && it.align == ALIGN16) {
+    if (full && it.align != ALIGN2) {
+        // 16-byte row stores (dword alignment suffices, as for the loads)
 #pragma unroll
         for (int r = 0; r < 8; r++)
             *(uint4 *)(p + (uint64_t)r * it.z) = make_uint4(D[4 * r], D[4 * r + 1], D[4 * r + 2], D[4 * r + 3]);
-    } else if (full && it.align == ALIGN4) {
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            uint32_t *q = (uint32_t *)(p + (uint64_t)r * it.z);
-#pragma unroll
-            for (int c = 0; c < 4; c++) q[c] = D[4 * r + c];
-        }
     } else {
 #pragma unroll
         for (int r = 0; r < 8; r++) {
@@ -148,6 +137,65 @@ __device__ __forceinline__ uint32_t tile_scan(uint32_t words, uint32_t tid, uint
     return before + incl - words;
 }
 
+// Copy `words` 4-byte words from LDS to global memory at dst (4-byte
+// aligned): dword stores up to 16-byte alignment, then 16-byte stores, then
+// the dword tail.
+template <int TILE>
+__device__ __forceinline__ void store_payload(const uint32_t *pay, uint32_t words, uint8_t *dst, uint32_t tid)
+{
+    uint32_t head = (uint32_t)((16 - ((uintptr_t)dst & 15)) & 15) >> 2;
+    if (head > words) head = words;
+    uint32_t *d32 = (uint32_t *)dst;
+    if (tid < head) d32[tid] = pay[tid];
+    const uint32_t nchunk = (words - head) >> 2;
+    uint4 *d128 = (uint4 *)(d32 + head);
+    for (uint32_t c = tid; c < nchunk; c += TILE) {
+        const uint32_t w = head + 4 * c;
+        d128[c] = make_uint4(pay[w], pay[w + 1], pay[w + 2], pay[w + 3]);
+    }
+    const uint32_t done = head + 4 * nchunk;
+    if (tid < words - done) d32[done + tid] = pay[done + tid];
+}
+
+// Stage `words` 4-byte words of a stream starting at byte g0 (4-byte aligned)
+// into LDS: 16-byte loads of the aligned chunks covering them, written to
+// pay[] from index 0 = the word at (g0 & ~15), so the first payload word sits at
+// pay[(g0 & 15) / 4].  Nothing at or past s_len is read (dword tail).
+template <int TILE>
+__device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len, uint64_t g0, uint32_t words,
+                                             uint32_t *pay, uint32_t tid)
+{
+    const uint64_t a0 = g0 & ~15ull;
+    const uint32_t shift = (uint32_t)(g0 - a0) >> 2;
+    const uint32_t nchunk = (shift + words + 3) >> 2;
+    const uint4 *src = (const uint4 *)(s_in + a0);
+    for (uint32_t base = 0; base < nchunk; base += 4u * TILE) {
+        uint4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {      // up to 4 loads in flight per lane
+            const uint32_t c = base + (uint32_t)k * TILE + tid;
+            v[k] = make_uint4(0, 0, 0, 0);
+            if (c < nchunk) {
+                if (a0 + 16ull * c + 16 <= s_len) {
+                    v[k] = src[c];
+                } else {
+                    const uint32_t *s32 = (const uint32_t *)(src + c);
+                    uint32_t t[4] = {0, 0, 0, 0};
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (a0 + 16ull * c + 4 * j + 4 <= s_len) t[j] = s32[j];
+                    v[k] = make_uint4(t[0], t[1], t[2], t[3]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t c = base + (uint32_t)k * TILE + tid;
+            if (c < nchunk) *(uint4 *)(pay + 4 * c) = v[k];
+        }
+    }
+}
+
 // LDS of the workgroup-wide look-back (NW waves).
 template <int NW>
 struct LbShared {
@@ -158,7 +206,7 @@ struct LbShared {
 template <int TILE>
 struct TileShared {
     static constexpr int kWaves = TILE / 64;
-    uint32_t pay[TILE * 32];       // payload staging, 4-byte words (128 B per block max)
+    alignas(16) uint32_t pay[TILE * 32 + 8];   // payload staging (128 B per block max) + shift slack
     uint32_t hdr[TILE / 4];        // header bytes of the tile
     uint32_t wsum[kWaves];
     LbShared<kWaves> lb;
@@ -252,10 +300,10 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
     }
 }
 
-// Dynamic tile ids.  first_tile() takes the workgroup's first id; at the top
-// of each iteration prefetch_tile() already takes the next one, so the atomic's
-// latency overlaps the current tile; advance_tile() (between barriers) makes
-// the prefetched id current.
+// Dynamic tile ids.  first_tile() takes an id for immediate processing.  Only
+// where no tile waits for another (decode with precomputed offsets) is the next
+// id taken early with prefetch_tile() and made current with advance_tile(): a
+// reserved tile must never hold up a look-back.
 template <class SH>
 __device__ __forceinline__ void first_tile(SH &sh, Ctrl *ctrl)
 {
@@ -304,52 +352,33 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
-    // Software pipeline: D holds the block of the current tile; the loads of
-    // the next tile are issued as soon as the current one is packed, so they
-    // are in flight during the look-back and the stores.
-    uint32_t D[32];
-    if (sh.tile < ntot) {
-        const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
-        const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
-        if (b < it.nblocks) {
-            uint32_t bx, by, bz;
-            block_coords(it, b, bx, by, bz);
-            load_block(it, bx, by, bz, D);
-        }
-    }
-
+    // Tile ids are taken one at a time, right before a tile is processed: a
+    // taken tile publishes its aggregate without waiting for any look-back, so
+    // successors never wait on a reserved but unstarted tile.
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
-        prefetch_tile(sh, ctrl);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
         const uint32_t b = tl * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
 
+        uint32_t D[32];
         uint32_t hdr = 0, words = 0;
         if (valid) {
+            uint32_t bx, by, bz;
+            block_coords(it, b, bx, by, bz);
+            load_block(it, bx, by, bz, D);
             hdr = block_header(D, it.dec);
             words = header_payload_bytes(hdr) >> 2;
         }
         ((uint8_t *)sh.hdr)[tid] = (uint8_t)hdr;
 
         uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);   // barrier: sh.next is visible
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
         if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
 
         if (words) encode_block(D, hdr, it.dec, &sh.pay[off]);
-
-        const uint32_t ntile = sh.next;
-        if (ntile < ntot) {
-            const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
-            const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
-            if (nb < nit.nblocks) {
-                uint32_t bx, by, bz;
-                block_coords(nit, nb, bx, by, bz);
-                load_block(nit, bx, by, bz, D);
-            }
-        }
 
         uint64_t excl = 0;
         if (tl != 0) {
@@ -358,19 +387,24 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
         }
         __syncthreads();   // every lane's payload is in LDS before the copy below
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
-        // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0)
-        for (uint32_t w = tid; w < (uint32_t)TILE / 4; w += TILE) {
-            const uint64_t o = (uint64_t)tl * TILE + 4ull * w;
-            if (o < it.hdr_bytes) *(uint32_t *)(it.s_out + o) = sh.hdr[w];
+        // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0):
+        // 16-byte stores, the last partial chunk by dwords
+        if (tid < (uint32_t)TILE / 16) {
+            const uint64_t o = (uint64_t)tl * TILE + 16ull * tid;
+            if (o + 16 <= it.hdr_bytes) {
+                *(uint4 *)(it.s_out + o) = make_uint4(sh.hdr[4 * tid], sh.hdr[4 * tid + 1], sh.hdr[4 * tid + 2],
+                                                      sh.hdr[4 * tid + 3]);
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < 4; k++)
+                    if (o + 4 * k < it.hdr_bytes) *(uint32_t *)(it.s_out + o + 4 * k) = sh.hdr[4 * tid + k];
+            }
         }
-        if (g0 + 4ull * agg <= it.s_len) {
-            uint32_t *dst = (uint32_t *)(it.s_out + g0);
-            for (uint32_t w = tid; w < agg; w += TILE) dst[w] = sh.pay[w];
-        }
+        if (g0 + 4ull * agg <= it.s_len) store_payload<TILE>(sh.pay, agg, it.s_out + g0, tid);
         if (tid == 0 && tl == it.ntiles - 1 && it.result) *it.result = g0 + 4ull * agg;
 
         __syncthreads();
-        advance_tile(sh);
+        first_tile(sh, ctrl);   // the next tile id
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
@@ -396,25 +430,28 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
-    // Pipeline: the header byte and (with precomputed offsets) the payload
-    // offset of the next tile are loaded while the current tile decodes.
+    // With precomputed tile offsets (no look-back) the header byte and the
+    // offset of the next tile are loaded while the current tile decodes; the
+    // next tile id is then taken early, which is safe because no tile waits
+    // for another.  With the look-back, ids are taken one at a time.
     uint32_t hdr = 0;
     uint64_t excl_next = 0;
-    if (sh.tile < ntot) {
+    if (tile_off && sh.tile < ntot) {
         const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
         const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
         hdr = b < it.nblocks ? (uint32_t)it.s_in[b] : 0u;
-        if (tile_off) excl_next = tile_off[sh.tile];
+        excl_next = tile_off[sh.tile];
     }
 
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
-        prefetch_tile(sh, ctrl);
+        if (tile_off) prefetch_tile(sh, ctrl);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
         const uint32_t b = tl * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
+        if (!tile_off) hdr = valid ? (uint32_t)it.s_in[b] : 0u;
 
         uint64_t excl = excl_next;   // with precomputed tile offsets no look-back is needed
         const uint32_t words = header_payload_bytes(hdr) >> 2;
@@ -432,26 +469,12 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
         }
         const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
         const bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
-        if (fits) {
-            // stage the tile's payload: 8 loads in flight per lane before the LDS writes
-            const uint32_t *src = (const uint32_t *)(it.s_in + g0);
-            for (uint32_t base = 0; base < agg; base += 8u * TILE) {
-                uint32_t v[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const uint32_t w = base + (uint32_t)k * TILE + tid;
-                    v[k] = w < agg ? src[w] : 0u;
-                }
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const uint32_t w = base + (uint32_t)k * TILE + tid;
-                    if (w < agg) sh.pay[w] = v[k];
-                }
-            }
-        }
+        // the tile's payload lands in LDS at word `shift` (g0 mod 16, in words)
+        const uint32_t shift = (uint32_t)(g0 & 15) >> 2;
+        if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
         // next tile's header byte and offset
         const uint32_t hdr_cur = hdr;
-        const uint32_t ntile = sh.next;
+        const uint32_t ntile = tile_off ? sh.next : ntot;
         if (ntile < ntot) {
             const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
             const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
@@ -464,7 +487,7 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
             uint32_t D[32];
             bad = bad || !fits;
             if (!bad && (hdr_cur & 0x1fu)) {
-                decode_block(&sh.pay[off], hdr_cur, it.dec.b, D);
+                decode_block(&sh.pay[shift + off], hdr_cur, it.dec.b, D);
             } else {
 #pragma unroll
                 for (int i = 0; i < 32; i++) D[i] = 0;
@@ -476,7 +499,8 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
                                            __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        advance_tile(sh);
+        if (tile_off) advance_tile(sh);
+        else first_tile(sh, ctrl);
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
@@ -522,7 +546,6 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
     for (;;) {
         const uint32_t chunk = sh.tile;
         if (chunk >= nchunks) break;
-        prefetch_tile(sh, ctrl);
         const uint32_t j = tid >> 2, q = tid & 3u;
         const uint32_t tile = chunk * 64 + j;
         const uint64_t first = (uint64_t)tile * TILE + q * kBytes;   // first header byte of this thread
@@ -566,7 +589,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
             if (t == it.ntiles - 1) tile_off[it.ntiles] = excl + incl;
         }
         __syncthreads();
-        advance_tile(sh);
+        first_tile(sh, ctrl);
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
