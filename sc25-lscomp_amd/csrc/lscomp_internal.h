@@ -17,17 +17,21 @@ enum : uint32_t { ALIGN2 = 0, ALIGN4 = 1, ALIGN16 = 2 };
 // Launch-state block of a scratch area.  All fields start at zero; the last
 // workgroup of every launch resets the counters and advances the epoch, so a
 // launch needs no host-side clearing (and replays inside a hipGraph).
+// Each field that many workgroups touch sits on its own 128-byte line: the
+// tile counter takes one atomic per tile and must not share its line with
+// anything else that is read or written concurrently.
 struct Ctrl {
     uint32_t tile_ctr;   // dynamic tile ids
+    uint32_t pad0[31];
     uint32_t done_ctr;   // workgroups finished
-    uint32_t epoch;      // tags look-back words of the current launch
-    uint32_t pad0;
+    uint32_t epoch;      // tags look-back words of the current launch (read once per workgroup)
+    uint32_t pad1[30];
     uint32_t error;      // sticky ERRF_* bits, cleared by the host
-    uint32_t pad1;
+    uint32_t pad2;
     uint64_t result;     // stream bytes of a single-item compression
-    uint64_t pad2[4];
+    uint64_t pad3[14];
 };
-static_assert(sizeof(Ctrl) == 64, "Ctrl is one 64-byte record");
+static_assert(sizeof(Ctrl) == 384, "Ctrl is three 128-byte lines");
 enum : uint32_t { ERRF_CORRUPT = 1u, ERRF_TIMEOUT = 2u };
 
 // Look-back word: epoch (26 bits) | flag (2 bits) | value in 4-byte words (36 bits).

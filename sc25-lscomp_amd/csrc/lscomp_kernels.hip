@@ -236,7 +236,11 @@ __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t firs
             const uint64_t incl_m = __ballot(flag == LB_INCL);
             const uint64_t empty_m = __ballot(flag == LB_EMPTY);
             if (lane == 0) { sh.incl[wave] = incl_m; sh.empty[wave] = empty_m; }
-            if (tid == 0) sh.err = __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the launch-wide error word is polled only while waiting, every 64
+            // probes: it must not become a hot spot for every look-back
+            if (tid == 0)
+                sh.err = (spins & 63u) == 63u
+                    ? __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
             __syncthreads();
             bool retry = false;
             j = TILE;
