@@ -207,7 +207,7 @@ template <int TILE>
 struct TileShared {
     static constexpr int kWaves = TILE / 64;
     alignas(16) uint32_t pay[TILE * 32 + 8];   // payload staging (128 B per block max) + shift slack
-    uint32_t hdr[TILE / 4];        // header bytes of the tile
+    uint32_t hdr[2][TILE / 4];     // header bytes, one buffer per pipeline stage
     uint32_t wsum[kWaves];
     LbShared<kWaves> lb;
     uint32_t tile, next, epoch, ntot;
@@ -217,21 +217,32 @@ struct TileShared {
 // its item: decoupled look-back with a window of TILE predecessors per probe,
 // one per thread of the workgroup.  Called by every thread (uniform control
 // flow, barriers inside).
+// First probe of a look-back: this thread's predecessor word (tile - 1 - tid),
+// issued early so that its latency overlaps other work.
+__device__ __forceinline__ uint64_t lb_first_probe(const uint64_t *status, uint32_t tile, uint32_t first,
+                                                   uint32_t epoch)
+{
+    const int64_t idx = (int64_t)tile - 1 - (int64_t)threadIdx.x;
+    return idx >= (int64_t)first ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
+}
+
 template <int TILE>
 __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t first, uint32_t epoch,
-                                  LbShared<TILE / 64> &sh, Ctrl *ctrl)
+                                  LbShared<TILE / 64> &sh, Ctrl *ctrl, bool have_w0 = false, uint64_t w0 = 0)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint64_t excl = 0;
     int64_t base = (int64_t)tile - 1;
     uint32_t spins = 0;
+    bool use_w0 = have_w0;   // the first probe of the first window was pre-issued
     for (;;) {
         const int64_t idx = base - (int64_t)tid;
         const bool inr = idx >= (int64_t)first;
         uint64_t w;
         uint32_t j;   // nearest inclusive predecessor, as a window position (TILE: none)
         for (;;) {
-            w = inr ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
+            w = use_w0 ? w0 : inr ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
+            use_w0 = false;
             const uint32_t flag = ((uint32_t)(w >> 38) == epoch) ? (uint32_t)(w >> 36) & 3u : 0u;
             const uint64_t incl_m = __ballot(flag == LB_INCL);
             const uint64_t empty_m = __ballot(flag == LB_EMPTY);
@@ -340,6 +351,22 @@ __device__ __forceinline__ const Item &tile_item(const Item &single, const Item 
 // ---------------------------------------------------------------------------
 // Compression.
 // ---------------------------------------------------------------------------
+template <bool kTable>
+__device__ __forceinline__ const Item &item_at(const Item &single, const Item *items, uint32_t idx)
+{
+    if (!kTable) return single;
+    return items[idx];
+}
+
+// Two-stage pipeline per workgroup.  Iteration k holds tile c (packed in LDS,
+// aggregate published, look-back pending) and takes on tile n:
+//   loads of n + first look-back probe of c  (in flight together)
+//   header/scan of n, publish n's aggregate  (n never waits for other tiles)
+//   finish c's look-back, publish c's inclusive prefix, take the next id
+//   store c's headers and payload from LDS, then pack n into LDS.
+// A tile id is taken only after this workgroup's look-back waits are over, so
+// between taking an id and publishing its aggregate a workgroup does only its
+// own work: no look-back ever waits on a reserved, unstarted tile.
 template <int TILE, bool kTable>
 __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const Item *__restrict__ items,
                                                         const uint32_t *__restrict__ tile_items,
@@ -351,65 +378,96 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
     if (tid == 0) {
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
+        sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    first_tile(sh, ctrl);
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
-    // Tile ids are taken one at a time, right before a tile is processed: a
-    // taken tile publishes its aggregate without waiting for any look-back, so
-    // successors never wait on a reserved but unstarted tile.
+    bool have_cur = false;
+    uint32_t c_tile = 0, c_tl = 0, c_item = 0, c_agg = 0, hb = 0;
     for (;;) {
-        const uint32_t tile = sh.tile;
-        if (tile >= ntot) break;
-        const Item &it = tile_item<kTable>(single, items, tile_items, tile);
-        const uint32_t tl = tile - it.first_tile;
-        const uint32_t b = tl * (uint32_t)TILE + tid;
-        const bool valid = b < it.nblocks;
+        const uint32_t n = sh.next;
+        const bool have_n = n < ntot;
+        if (!have_n && !have_cur) break;
 
+        // stage 1: loads of n, first probe of c
         uint32_t D[32];
-        uint32_t hdr = 0, words = 0;
-        if (valid) {
-            uint32_t bx, by, bz;
-            block_coords(it, b, bx, by, bz);
-            load_block(it, bx, by, bz, D);
-            hdr = block_header(D, it.dec);
-            words = header_payload_bytes(hdr) >> 2;
-        }
-        ((uint8_t *)sh.hdr)[tid] = (uint8_t)hdr;
-
-        uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
-        if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
-
-        if (words) encode_block(D, hdr, it.dec, &sh.pay[off]);
-
-        uint64_t excl = 0;
-        if (tl != 0) {
-            excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
-            if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
-        }
-        __syncthreads();   // every lane's payload is in LDS before the copy below
-        const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
-        // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0):
-        // 16-byte stores, the last partial chunk by dwords
-        if (tid < (uint32_t)TILE / 16) {
-            const uint64_t o = (uint64_t)tl * TILE + 16ull * tid;
-            if (o + 16 <= it.hdr_bytes) {
-                *(uint4 *)(it.s_out + o) = make_uint4(sh.hdr[4 * tid], sh.hdr[4 * tid + 1], sh.hdr[4 * tid + 2],
-                                                      sh.hdr[4 * tid + 3]);
-            } else {
-#pragma unroll
-                for (uint32_t k = 0; k < 4; k++)
-                    if (o + 4 * k < it.hdr_bytes) *(uint32_t *)(it.s_out + o + 4 * k) = sh.hdr[4 * tid + k];
+        uint32_t n_item = 0, n_tl = 0, nb = 0;
+        bool nvalid = false;
+        if (have_n) {
+            n_item = kTable ? __builtin_amdgcn_readfirstlane(tile_items[n]) : 0u;
+            const Item &nit = item_at<kTable>(single, items, n_item);
+            n_tl = n - nit.first_tile;
+            nb = n_tl * (uint32_t)TILE + tid;
+            nvalid = nb < nit.nblocks;
+            if (nvalid) {
+                uint32_t bx, by, bz;
+                block_coords(nit, nb, bx, by, bz);
+                load_block(nit, bx, by, bz, D);
             }
         }
-        if (g0 + 4ull * agg <= it.s_len) store_payload<TILE>(sh.pay, agg, it.s_out + g0, tid);
-        if (tid == 0 && tl == it.ntiles - 1 && it.result) *it.result = g0 + 4ull * agg;
+        uint64_t w0 = 0;
+        if (have_cur && c_tl != 0)
+            w0 = lb_first_probe(status, c_tile, item_at<kTable>(single, items, c_item).first_tile, epoch);
 
-        __syncthreads();
-        first_tile(sh, ctrl);   // the next tile id
-        __syncthreads();
+        // stage 2: header, scan and aggregate of n
+        uint32_t n_hdr = 0, n_words = 0, n_off = 0, n_agg = 0;
+        if (have_n) {
+            const Item &nit = item_at<kTable>(single, items, n_item);
+            if (nvalid) {
+                n_hdr = block_header(D, nit.dec);
+                n_words = header_payload_bytes(n_hdr) >> 2;
+            }
+            ((uint8_t *)sh.hdr[hb])[tid] = (uint8_t)n_hdr;
+            n_off = tile_scan<TILE>(n_words, tid, sh.wsum, &n_agg);
+            if (tid == 0) st_relaxed(status + n, lb_word(epoch, n_tl == 0 ? LB_INCL : LB_AGG, n_agg));
+        }
+
+        // stage 3: finish c
+        if (have_cur) {
+            const Item &cit = item_at<kTable>(single, items, c_item);
+            uint64_t excl = 0;
+            if (c_tl != 0) {
+                excl = lookback_tile<TILE>(status, c_tile, cit.first_tile, epoch, sh.lb, ctrl, true, w0);
+                if (tid == 0) st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, excl + c_agg));
+            }
+            if (tid == 0)   // no waits remain in this iteration: take the next id
+                sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t *chdr = sh.hdr[hb ^ 1u];
+            const uint64_t g0 = (uint64_t)cit.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
+            // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0):
+            // 16-byte stores, the last partial chunk by dwords
+            if (tid < (uint32_t)TILE / 16) {
+                const uint64_t o = (uint64_t)c_tl * TILE + 16ull * tid;
+                if (o + 16 <= cit.hdr_bytes) {
+                    *(uint4 *)(cit.s_out + o) = make_uint4(chdr[4 * tid], chdr[4 * tid + 1], chdr[4 * tid + 2],
+                                                           chdr[4 * tid + 3]);
+                } else {
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; k++)
+                        if (o + 4 * k < cit.hdr_bytes) *(uint32_t *)(cit.s_out + o + 4 * k) = chdr[4 * tid + k];
+                }
+            }
+            if (g0 + 4ull * c_agg <= cit.s_len) store_payload<TILE>(sh.pay, c_agg, cit.s_out + g0, tid);
+            if (tid == 0 && c_tl == cit.ntiles - 1 && cit.result) *cit.result = g0 + 4ull * c_agg;
+        } else if (tid == 0) {
+            sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();   // c's payload has left LDS; sh.next is visible
+
+        // stage 4: pack n; it becomes c
+        if (have_n) {
+            if (n_words) encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_off]);
+            have_cur = true;
+            c_tile = n;
+            c_tl = n_tl;
+            c_item = n_item;
+            c_agg = n_agg;
+            hb ^= 1u;
+        } else {
+            have_cur = false;
+        }
+        __syncthreads();   // n's payload is in LDS before the next iteration stores it
     }
     finish_launch(ctrl, epoch);
 }
