@@ -100,7 +100,8 @@ void lscomp_release_all(void);
 
 /* Launch shape of later calls (process-wide): direction 0 = compression,
  * 1 = decompression; tile = 8x8 blocks per workgroup (128, 256 or 512;
- * 0 = default 256); wg_per_cu = workgroups per CU (0 = occupancy limit).
+ * 0 = the default, 512); wg_per_cu =
+ * workgroups per CU (0 = occupancy limit).
  * Streams are identical for every shape; only speed changes.  No counterpart
  * in the reference (its grid is fixed, cuLSZ_entry.cu:100-101); set by the
  * autotuner or by hand. */

@@ -48,7 +48,7 @@ uint64_t num_blocks(lscomp_dim3 d)
 
 // Launch shape per direction (0 compress, 1 decompress): tile and workgroups
 // per CU (0 = occupancy limit).  Set by lscomp_set_launch_shape / the autotuner.
-std::atomic<uint32_t> g_tile[2] = {{(uint32_t)kDefaultTile}, {(uint32_t)kDefaultTile}};
+std::atomic<uint32_t> g_tile[2] = {{kDefaultTile[0]}, {kDefaultTile[1]}};
 std::atomic<uint32_t> g_wgpcu[2] = {{0u}, {0u}};
 // Single-volume decompression computes tile offsets in a pre-pass instead of
 // a look-back (LSCOMP_DECODE_PREPASS=0 in the environment turns it off).
@@ -394,7 +394,7 @@ void lscomp_release_all(void)
 int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu)
 {
     if (direction != 0 && direction != 1) return LSCOMP_ERR_INVALID_ARG;
-    if (tile == 0) tile = kDefaultTile;
+    if (tile == 0) tile = kDefaultTile[direction];
     if (tile != 128 && tile != 256 && tile != 512) return LSCOMP_ERR_INVALID_ARG;
     g_tile[direction].store(tile);
     g_wgpcu[direction].store(wg_per_cu);

@@ -18,7 +18,7 @@ using namespace lsc;
 struct lscomp_plan_s {
     int device = 0;
     int direction = 0;            // 0 compress, 1 decompress
-    uint32_t tile = kDefaultTile, wgpcu = 0;
+    uint32_t tile = kDefaultTile[0], wgpcu = 0;
     uint32_t max_items = 0;
     uint64_t max_tiles = 0;
     uint32_t nitems = 0, ntiles = 0;

@@ -4,17 +4,17 @@
 // item, one block per lane; persistent workgroups loop over tiles.
 // Compression (replaces cuLSZ_compression_kernel_uint16_bsize64,
 // lsCOMP/src/cuLSZ_kernel.cu:827-1283):
-//   1. dynamic tile ids (forward progress does not depend on dispatch order),
-//      the next id prefetched while the current tile runs;
-//   2. each lane loads its block once (8 row loads, coalesced across lanes:
-//      lanes are consecutive blocks along z), computes the header byte;
+//   1. dynamic tile ids (forward progress does not depend on dispatch order);
+//   2. each lane loads its block once (8 16-byte row loads, coalesced across
+//      lanes: lanes are consecutive blocks along z), computes the header byte;
 //   3. workgroup scan of payload sizes, tile aggregate published at once;
-//   4. payload packed into LDS while predecessors resolve;
-//   5. decoupled look-back with a window of TILE predecessors per probe (one
-//      per thread), one 64-bit word per tile (epoch | flag | value) read and
-//      written with relaxed agent-scope atomics, so no fences are needed on
-//      the multi-XCD part;
-//   6. headers and payload leave LDS as contiguous coalesced dword stores.
+//   4. two tiles in flight per workgroup: the loads of the next tile overlap
+//      the look-back of the current one, whose payload waits packed in LDS;
+//   5. decoupled look-back with a window of kLbPer * TILE predecessors per
+//      probe round (kLbPer per thread), one 64-bit word per tile (epoch | flag
+//      | value) read and written with relaxed agent-scope atomics, so no
+//      fences are needed on the multi-XCD part;
+//   6. headers and payload leave LDS as contiguous coalesced 16-byte stores.
 // Decompression (replaces cuLSZ_decompression_kernel_uint16_bsize64,
 // cuLSZ_kernel.cu:1286-1646) runs the same scan over header sizes, stages the
 // tile's payload in LDS with coalesced loads, and each lane unpacks its block
@@ -65,26 +65,48 @@ __device__ __forceinline__ void load_block(const Item &it, uint32_t bx, uint32_t
 {
     const uint32_t y0 = by * 8, z0 = bz * 8;
     const uint16_t *p = it.vol_in + ((uint64_t)bx * it.y + y0) * it.z + z0;
-    const bool full = (y0 + 8 <= it.y) && (z0 + 8 <= it.z);
-    if (full && it.align != ALIGN2) {
-        // 16-byte row loads; global_load_dwordx4 needs only dword alignment,
-        // so rows of 4-byte-aligned volumes (z even) load whole as well
+    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;   // rows inside the volume
+    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;   // columns inside the volume
+    if (it.align != ALIGN2) {
+        // 16-byte row loads; global_load_dwordx4 needs only dword alignment, so
+        // rows of 4-byte-aligned volumes (z even) load whole.  A block at the
+        // end of a row (nz < 8, even) loads 16 bytes as well while they lie
+        // inside the volume (the excess is the next row's start) and clears the
+        // pairs past z; only the volume's last row falls back to dword loads.
+        const uint16_t *vol_end = it.vol_in + (uint64_t)it.x * it.y * it.z;
+        const uint32_t keep = nz >> 1;   // dwords of a row inside the volume
         uint4 t[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) t[r] = *(const uint4 *)(p + (uint64_t)r * it.z);
+        for (int r = 0; r < 8; r++) {
+            const uint16_t *q = p + (uint64_t)r * it.z;
+            t[r] = make_uint4(0, 0, 0, 0);
+            if ((uint32_t)r < ny) {
+                if (nz == 8u || q + 8 <= vol_end) {
+                    t[r] = *(const uint4 *)q;
+                } else {
+                    const uint32_t *q32 = (const uint32_t *)q;
+                    t[r].x = q32[0];   // keep >= 1 (nz is even and nonzero)
+                    if (keep > 1) t[r].y = q32[1];
+                    if (keep > 2) t[r].z = q32[2];
+                }
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            D[4 * r] = t[r].x; D[4 * r + 1] = t[r].y; D[4 * r + 2] = t[r].z; D[4 * r + 3] = t[r].w;
+            D[4 * r] = t[r].x;
+            D[4 * r + 1] = keep > 1 ? t[r].y : 0u;
+            D[4 * r + 2] = keep > 2 ? t[r].z : 0u;
+            D[4 * r + 3] = keep > 3 ? t[r].w : 0u;
         }
     } else {
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            const bool rv = y0 + r < it.y;
+            const bool rv = (uint32_t)r < ny;
             const uint16_t *q = p + (uint64_t)r * it.z;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const uint32_t lo = (rv && z0 + 2 * c < it.z) ? q[2 * c] : 0u;
-                const uint32_t hi = (rv && z0 + 2 * c + 1 < it.z) ? q[2 * c + 1] : 0u;
+                const uint32_t lo = (rv && 2u * c < nz) ? q[2 * c] : 0u;
+                const uint32_t hi = (rv && 2u * c + 1 < nz) ? q[2 * c + 1] : 0u;
                 D[4 * r + c] = lo | (hi << 16);
             }
         }
@@ -96,21 +118,33 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
 {
     const uint32_t y0 = by * 8, z0 = bz * 8;
     uint16_t *p = it.vol_out + ((uint64_t)bx * it.y + y0) * it.z + z0;
-    const bool full = (y0 + 8 <= it.y) && (z0 + 8 <= it.z);
-    if (full && it.align != ALIGN2) {
-        // 16-byte row stores (dword alignment suffices, as for the loads)
+    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
+    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
+    if (it.align != ALIGN2) {
+        // 16-byte row stores (dword alignment suffices, as for the loads); a
+        // block at the end of a row stores its nz/2 dwords
 #pragma unroll
-        for (int r = 0; r < 8; r++)
-            *(uint4 *)(p + (uint64_t)r * it.z) = make_uint4(D[4 * r], D[4 * r + 1], D[4 * r + 2], D[4 * r + 3]);
+        for (int r = 0; r < 8; r++) {
+            if ((uint32_t)r >= ny) break;
+            uint16_t *q = p + (uint64_t)r * it.z;
+            if (nz == 8u) {
+                *(uint4 *)q = make_uint4(D[4 * r], D[4 * r + 1], D[4 * r + 2], D[4 * r + 3]);
+            } else {
+                uint32_t *q32 = (uint32_t *)q;
+                q32[0] = D[4 * r];
+                if (nz > 2u) q32[1] = D[4 * r + 1];
+                if (nz > 4u) q32[2] = D[4 * r + 2];
+            }
+        }
     } else {
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            if (y0 + r >= it.y) break;
+            if ((uint32_t)r >= ny) break;
             uint16_t *q = p + (uint64_t)r * it.z;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                if (z0 + 2 * c < it.z) q[2 * c] = (uint16_t)(D[4 * r + c] & 0xffffu);
-                if (z0 + 2 * c + 1 < it.z) q[2 * c + 1] = (uint16_t)(D[4 * r + c] >> 16);
+                if (2u * c < nz) q[2 * c] = (uint16_t)(D[4 * r + c] & 0xffffu);
+                if (2u * c + 1 < nz) q[2 * c + 1] = (uint16_t)(D[4 * r + c] >> 16);
             }
         }
     }
@@ -196,10 +230,22 @@ __device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len
     }
 }
 
-// LDS of the workgroup-wide look-back (NW waves).
+// Look-back window: each thread reads kLbPer predecessor words per probe
+// round, so one round covers kLbPer * TILE tiles.  The nearest inclusive
+// prefix typically lies about one pipeline depth of all workgroups back
+// (~2 x grid tiles), so a window of several tiles per thread usually resolves
+// in the first, pre-issued round.
+#ifndef LSC_LBPER
+#define LSC_LBPER 4
+#endif
+constexpr int kLbPer = LSC_LBPER;
+
+// LDS of the workgroup-wide look-back (NW waves): per wave, the window
+// positions of its nearest inclusive and nearest empty predecessor.
 template <int NW>
 struct LbShared {
-    uint64_t incl[NW], empty[NW], sum[NW];
+    uint32_t pos_incl[NW], pos_empty[NW];
+    uint64_t sum[NW];
     uint32_t err;
 };
 
@@ -213,59 +259,74 @@ struct TileShared {
     uint32_t tile, next, epoch, ntot;
 };
 
-// Exclusive prefix (in words) of tile `tile` over the tiles [first, tile) of
-// its item: decoupled look-back with a window of TILE predecessors per probe,
-// one per thread of the workgroup.  Called by every thread (uniform control
-// flow, barriers inside).
-// First probe of a look-back: this thread's predecessor word (tile - 1 - tid),
-// issued early so that its latency overlaps other work.
-__device__ __forceinline__ uint64_t lb_first_probe(const uint64_t *status, uint32_t tile, uint32_t first,
-                                                   uint32_t epoch)
+// One probe round: this thread's words of the predecessors at distances
+// 1 + k*TILE + tid (k < kLbPer) behind `base + 1`; tiles before `first` (the
+// item's first tile) read as an inclusive prefix of 0.
+template <int TILE>
+__device__ __forceinline__ void lb_probe(const uint64_t *status, int64_t base, uint32_t first, uint32_t epoch,
+                                         uint64_t w[kLbPer])
 {
-    const int64_t idx = (int64_t)tile - 1 - (int64_t)threadIdx.x;
-    return idx >= (int64_t)first ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
+#pragma unroll
+    for (int k = 0; k < kLbPer; k++) {
+        const int64_t idx = base - (int64_t)k * TILE - (int64_t)threadIdx.x;
+        w[k] = idx >= (int64_t)first ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
+    }
 }
 
+// Exclusive prefix (in words) of tile `tile` over the tiles [first, tile) of
+// its item: decoupled look-back over windows of kLbPer * TILE predecessors.
+// Called by every thread (uniform control flow, barriers inside).  With
+// have_w0 the first round's words were read earlier by lb_probe(tile - 1)
+// (so that their latency overlaps other work) and are passed in w0.
 template <int TILE>
 __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t first, uint32_t epoch,
-                                  LbShared<TILE / 64> &sh, Ctrl *ctrl, bool have_w0 = false, uint64_t w0 = 0)
+                                  LbShared<TILE / 64> &sh, Ctrl *ctrl, bool have_w0, const uint64_t *w0)
 {
+    constexpr uint32_t kWin = (uint32_t)kLbPer * TILE;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint64_t excl = 0;
     int64_t base = (int64_t)tile - 1;
     uint32_t spins = 0;
-    bool use_w0 = have_w0;   // the first probe of the first window was pre-issued
+    bool use_w0 = have_w0;
     for (;;) {
-        const int64_t idx = base - (int64_t)tid;
-        const bool inr = idx >= (int64_t)first;
-        uint64_t w;
-        uint32_t j;   // nearest inclusive predecessor, as a window position (TILE: none)
+        uint64_t w[kLbPer];
+        uint32_t j;   // nearest inclusive predecessor as a window position (kWin: none)
         for (;;) {
-            w = use_w0 ? w0 : inr ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
+            if (use_w0) {
+#pragma unroll
+                for (int k = 0; k < kLbPer; k++) w[k] = w0[k];
+            } else {
+                lb_probe<TILE>(status, base, first, epoch, w);
+            }
             use_w0 = false;
-            const uint32_t flag = ((uint32_t)(w >> 38) == epoch) ? (uint32_t)(w >> 36) & 3u : 0u;
-            const uint64_t incl_m = __ballot(flag == LB_INCL);
-            const uint64_t empty_m = __ballot(flag == LB_EMPTY);
-            if (lane == 0) { sh.incl[wave] = incl_m; sh.empty[wave] = empty_m; }
+            // window position of this thread's k-th word: k * TILE + tid; per
+            // wave, the nearest inclusive and the nearest empty position
+            uint32_t pi = kWin, pe = kWin;
+#pragma unroll
+            for (int k = kLbPer - 1; k >= 0; k--) {
+                const uint32_t flag = ((uint32_t)(w[k] >> 38) == epoch) ? (uint32_t)(w[k] >> 36) & 3u : 0u;
+                const uint64_t im = __ballot(flag == LB_INCL), em = __ballot(flag == LB_EMPTY);
+                const uint32_t at = (uint32_t)k * TILE + (tid & ~63u);
+                if (im) pi = at + (uint32_t)__builtin_ctzll(im);
+                if (em) pe = at + (uint32_t)__builtin_ctzll(em);
+            }
+            if (lane == 0) { sh.pos_incl[wave] = pi; sh.pos_empty[wave] = pe; }
             // the launch-wide error word is polled only while waiting, every 64
             // probes: it must not become a hot spot for every look-back
             if (tid == 0)
                 sh.err = (spins & 63u) == 63u
                     ? __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
             __syncthreads();
-            bool retry = false;
-            j = TILE;
-#pragma unroll
+            j = kWin;
+            uint32_t e = kWin;
             for (int v = 0; v < TILE / 64; v++) {
-                const uint64_t im = sh.incl[v], em = sh.empty[v];
-                if (im) {
-                    const uint64_t need = ((im & (~im + 1ull)) << 1) - 1ull;
-                    retry = (em & need) != 0;
-                    j = 64u * v + (uint32_t)__builtin_ctzll(im);
-                    break;
-                }
-                if (em) { retry = true; break; }
+                const uint32_t a = sh.pos_incl[v], b = sh.pos_empty[v];
+                j = a < j ? a : j;
+                e = b < e ? b : e;
             }
+            // an empty predecessor nearer than the nearest inclusive one (or
+            // anywhere, when the window holds no inclusive one) must be re-read
+            const bool retry = e < j;
             const uint32_t err = sh.err;
             __syncthreads();
             if (!retry) break;
@@ -278,7 +339,10 @@ __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t firs
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        const uint64_t part = wave_sum_u64(tid <= j ? (w & kValMask) : 0ull);
+        uint64_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < kLbPer; k++) mine += ((uint32_t)k * TILE + tid <= j) ? (w[k] & kValMask) : 0ull;
+        const uint64_t part = wave_sum_u64(mine);
         if (lane == 0) sh.sum[wave] = part;
         __syncthreads();
         uint64_t total = 0;
@@ -286,8 +350,8 @@ __device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t firs
         for (int v = 0; v < TILE / 64; v++) total += sh.sum[v];
         __syncthreads();
         excl += total;
-        if (j < (uint32_t)TILE) return excl;
-        base -= TILE;
+        if (j < kWin) return excl;
+        base -= kWin;
     }
 }
 
@@ -368,7 +432,7 @@ __device__ __forceinline__ const Item &item_at(const Item &single, const Item *i
 // between taking an id and publishing its aggregate a workgroup does only its
 // own work: no look-back ever waits on a reserved, unstarted tile.
 template <int TILE, bool kTable>
-__global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const Item *__restrict__ items,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(const Item single, const Item *__restrict__ items,
                                                         const uint32_t *__restrict__ tile_items,
                                                         uint32_t total_tiles, const uint32_t *d_total_tiles,
                                                         Ctrl *ctrl, uint64_t *status)
@@ -406,9 +470,9 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
                 load_block(nit, bx, by, bz, D);
             }
         }
-        uint64_t w0 = 0;
+        uint64_t w0[kLbPer];
         if (have_cur && c_tl != 0)
-            w0 = lb_first_probe(status, c_tile, item_at<kTable>(single, items, c_item).first_tile, epoch);
+            lb_probe<TILE>(status, (int64_t)c_tile - 1, item_at<kTable>(single, items, c_item).first_tile, epoch, w0);
 
         // stage 2: header, scan and aggregate of n
         uint32_t n_hdr = 0, n_words = 0, n_off = 0, n_agg = 0;
@@ -476,7 +540,7 @@ __global__ __launch_bounds__(TILE) void compress_kernel(const Item single, const
 // Decompression.
 // ---------------------------------------------------------------------------
 template <int TILE, bool kTable>
-__global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, const Item *__restrict__ items,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_kernel(const Item single, const Item *__restrict__ items,
                                                           const uint32_t *__restrict__ tile_items,
                                                           uint32_t total_tiles, const uint32_t *d_total_tiles,
                                                           Ctrl *ctrl, uint64_t *status,
@@ -525,7 +589,7 @@ __global__ __launch_bounds__(TILE) void decompress_kernel(const Item single, con
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
             excl = 0;
             if (tl != 0) {
-                excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
+                excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl, false, nullptr);
                 if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
             }
         }
@@ -642,7 +706,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
         const uint32_t cagg = sh.excl_lo;
         uint64_t excl = 0;
         if (chunk != 0) {
-            excl = lookback_tile<256>(status, chunk, 0, epoch, sh.lb, ctrl);
+            excl = lookback_tile<256>(status, chunk, 0, epoch, sh.lb, ctrl, false, nullptr);
             if (tid == 0) st_relaxed(status + chunk, lb_word(epoch, LB_INCL, excl + cagg));
         }
         if (tid < 64) {

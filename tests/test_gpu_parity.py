@@ -38,8 +38,11 @@ def check_case(gpu, orc, vol, bins, p):
     return n
 
 
+# z even but not a multiple of 8 (1558, 10, 12, 2, 1030): blocks at the row
+# end load 16 bytes past z and clear the excess, except on the volume's last row
 SHAPES = [(1, 8, 8), (1, 1, 1), (1, 8, 9), (2, 13, 17), (1, 9, 9), (1, 8, 1), (3, 64, 64),
-          (1, 100, 101), (2, 50, 1558), (1, 1024, 1024), (16, 512, 512)]
+          (1, 100, 101), (2, 50, 1558), (1, 1024, 1024), (16, 512, 512),
+          (1, 9, 10), (2, 16, 12), (1, 8, 2), (3, 17, 1030)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -71,11 +74,14 @@ def test_long_lookback_chain(gpu, orc):
     check_case(gpu, orc, vol, (5, 5, 5, 5), 0.5)
 
 
+@pytest.mark.parametrize("shape", [(2, 64, 96), (2, 21, 30)], ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("offset_elems", [1, 2, 8])
-def test_misaligned_input(gpu, orc, offset_elems):
-    """Volumes at 2-, 4- and 16-byte offsets inside a larger buffer use other load paths."""
+def test_misaligned_input(gpu, orc, offset_elems, shape):
+    """Volumes at 2-, 4- and 16-byte offsets inside a larger buffer use other load
+    paths; with z = 30 the row-end blocks are partial.  Decoding leaves the
+    bytes around the volume untouched."""
     import torch
-    vol = synth("cssi", (2, 64, 96), seed=11)
+    vol = synth("cssi", shape, seed=11)
     big = torch.zeros(vol.size + 16, dtype=torch.int16, device="cuda")
     big[offset_elems:offset_elems + vol.size] = to_dev(vol).reshape(-1)
     d = big[offset_elems:offset_elems + vol.size]
