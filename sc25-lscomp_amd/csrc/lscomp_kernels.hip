@@ -10,9 +10,9 @@
 //   3. workgroup scan of payload sizes, tile aggregate published at once;
 //   4. two tiles in flight per workgroup: the loads of the next tile overlap
 //      the look-back of the current one, whose payload waits packed in LDS;
-//   5. decoupled look-back with a window of kLbPer * TILE predecessors per
-//      probe round (kLbPer per thread), one 64-bit word per tile (epoch | flag
-//      | value) read and written with relaxed agent-scope atomics, so no
+//   5. decoupled look-back with a window of about 1024 predecessors per
+//      probe round (1024/TILE per thread), one 64-bit word per tile (epoch |
+//      flag | value) read and written with relaxed agent-scope atomics, so no
 //      fences are needed on the multi-XCD part;
 //   6. headers and payload leave LDS as contiguous coalesced 16-byte stores.
 // Decompression (replaces cuLSZ_decompression_kernel_uint16_bsize64,
@@ -230,23 +230,22 @@ __device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len
     }
 }
 
-// Look-back window: each thread reads kLbPer predecessor words per probe
-// round, so one round covers kLbPer * TILE tiles.  The nearest inclusive
-// prefix typically lies about one pipeline depth of all workgroups back
-// (~2 x grid tiles), so a window of several tiles per thread usually resolves
-// in the first, pre-issued round.
-#ifndef LSC_LBPER
-#define LSC_LBPER 4
-#endif
-constexpr int kLbPer = LSC_LBPER;
+// Look-back window: each thread reads lb_per(TILE) predecessor words per
+// probe round, so one round covers about 1024 tiles.  The nearest inclusive
+// prefix typically lies about one tile per resident workgroup back, so the
+// first, pre-issued round usually resolves; a larger window costs uncached
+// status reads on every tile.
+constexpr int lb_per(int tile) { return tile >= 1024 ? 1 : (1024 / tile > 4 ? 4 : 1024 / tile); }
 
-// LDS of the workgroup-wide look-back (NW waves): per wave, the window
-// positions of its nearest inclusive and nearest empty predecessor.
-template <int NW>
+// LDS of the workgroup-wide look-back, per slice of 64 window positions (one
+// wave's k-th words): nearest inclusive and empty positions and the sum of the
+// values up to the nearest inclusive one (all values without one).  Two
+// buffers, used alternately, so one barrier per round suffices.
+template <int NSLICE>
 struct LbShared {
-    uint32_t pos_incl[NW], pos_empty[NW];
-    uint64_t sum[NW];
-    uint32_t err;
+    uint32_t pos_incl[2][NSLICE], pos_empty[2][NSLICE];
+    uint64_t sum[2][NSLICE];
+    uint32_t err[2];
 };
 
 template <int TILE>
@@ -255,104 +254,118 @@ struct TileShared {
     alignas(16) uint32_t pay[TILE * 32 + 8];   // payload staging (128 B per block max) + shift slack
     uint32_t hdr[2][TILE / 4];     // header bytes, one buffer per pipeline stage
     uint32_t wsum[kWaves];
-    LbShared<kWaves> lb;
+    LbShared<kWaves * lb_per(TILE)> lb;
     uint32_t tile, next, epoch, ntot;
 };
 
 // One probe round: this thread's words of the predecessors at distances
-// 1 + k*TILE + tid (k < kLbPer) behind `base + 1`; tiles before `first` (the
-// item's first tile) read as an inclusive prefix of 0.
+// 1 + k*TILE + tid (k < lb_per(TILE)) behind `base + 1`; tiles before `first`
+// (the item's first tile) read as an inclusive prefix of 0.
 template <int TILE>
 __device__ __forceinline__ void lb_probe(const uint64_t *status, int64_t base, uint32_t first, uint32_t epoch,
-                                         uint64_t w[kLbPer])
+                                         uint64_t w[lb_per(TILE)])
 {
 #pragma unroll
-    for (int k = 0; k < kLbPer; k++) {
+    for (int k = 0; k < lb_per(TILE); k++) {
         const int64_t idx = base - (int64_t)k * TILE - (int64_t)threadIdx.x;
         w[k] = idx >= (int64_t)first ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
     }
 }
 
-// Exclusive prefix (in words) of tile `tile` over the tiles [first, tile) of
-// its item: decoupled look-back over windows of kLbPer * TILE predecessors.
-// Called by every thread (uniform control flow, barriers inside).  With
-// have_w0 the first round's words were read earlier by lb_probe(tile - 1)
-// (so that their latency overlaps other work) and are passed in w0.
+enum : uint32_t { LB_RETRY = 0, LB_NEXT = 1, LB_DONE = 2 };
+
+// Evaluates one probe round (words w[] of the window that ends at the tile
+// being resolved minus base): adds the values up to the nearest inclusive
+// predecessor to excl.  Returns LB_DONE (excl is the exclusive prefix),
+// LB_NEXT (no inclusive predecessor in the window: continue one window further
+// back) or LB_RETRY (an empty predecessor lies before the nearest inclusive
+// one: read the same window again).  *err receives the launch error word when
+// `poll` is set (else 0).  `par` alternates between consecutive calls (LDS
+// buffer).  Called by every thread; one barrier.
 template <int TILE>
-__device__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t first, uint32_t epoch,
-                                  LbShared<TILE / 64> &sh, Ctrl *ctrl, bool have_w0, const uint64_t *w0)
+__device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint32_t epoch,
+                                            LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, bool poll,
+                                            uint32_t par, uint64_t &excl, uint32_t *err)
 {
-    constexpr uint32_t kWin = (uint32_t)kLbPer * TILE;
+    constexpr int P = lb_per(TILE), NW = TILE / 64;
+    constexpr uint32_t kWin = (uint32_t)P * TILE;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    uint64_t excl = 0;
-    int64_t base = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    bool use_w0 = have_w0;
-    for (;;) {
-        uint64_t w[kLbPer];
-        uint32_t j;   // nearest inclusive predecessor as a window position (kWin: none)
-        for (;;) {
-            if (use_w0) {
 #pragma unroll
-                for (int k = 0; k < kLbPer; k++) w[k] = w0[k];
-            } else {
-                lb_probe<TILE>(status, base, first, epoch, w);
-            }
-            use_w0 = false;
-            // window position of this thread's k-th word: k * TILE + tid; per
-            // wave, the nearest inclusive and the nearest empty position
-            uint32_t pi = kWin, pe = kWin;
-#pragma unroll
-            for (int k = kLbPer - 1; k >= 0; k--) {
-                const uint32_t flag = ((uint32_t)(w[k] >> 38) == epoch) ? (uint32_t)(w[k] >> 36) & 3u : 0u;
-                const uint64_t im = __ballot(flag == LB_INCL), em = __ballot(flag == LB_EMPTY);
-                const uint32_t at = (uint32_t)k * TILE + (tid & ~63u);
-                if (im) pi = at + (uint32_t)__builtin_ctzll(im);
-                if (em) pe = at + (uint32_t)__builtin_ctzll(em);
-            }
-            if (lane == 0) { sh.pos_incl[wave] = pi; sh.pos_empty[wave] = pe; }
-            // the launch-wide error word is polled only while waiting, every 64
-            // probes: it must not become a hot spot for every look-back
-            if (tid == 0)
-                sh.err = (spins & 63u) == 63u
-                    ? __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-            __syncthreads();
-            j = kWin;
-            uint32_t e = kWin;
-            for (int v = 0; v < TILE / 64; v++) {
-                const uint32_t a = sh.pos_incl[v], b = sh.pos_empty[v];
-                j = a < j ? a : j;
-                e = b < e ? b : e;
-            }
-            // an empty predecessor nearer than the nearest inclusive one (or
-            // anywhere, when the window holds no inclusive one) must be re-read
-            const bool retry = e < j;
-            const uint32_t err = sh.err;
-            __syncthreads();
-            if (!retry) break;
-            // Bounded wait: after kSpinLimit polls, or once any tile of the
-            // launch has given up, stop waiting (the launch is flagged bad).
-            if ((err & ERRF_TIMEOUT) || ++spins > kSpinLimit) {
-                if (tid == 0) __hip_atomic_fetch_or(&ctrl->error, ERRF_TIMEOUT, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-                return 0;
-            }
-            __builtin_amdgcn_s_sleep(1);
+    for (int k = 0; k < P; k++) {
+        const uint32_t flag = ((uint32_t)(w[k] >> 38) == epoch) ? (uint32_t)(w[k] >> 36) & 3u : 0u;
+        const uint64_t im = __ballot(flag == LB_INCL), em = __ballot(flag == LB_EMPTY);
+        // slice sum: values of the lanes up to the first inclusive one (all lanes without one)
+        const uint32_t cut = im ? (uint32_t)__builtin_ctzll(im) : 63u;
+        const uint64_t part = wave_sum_u64(lane <= cut ? (w[k] & kValMask) : 0ull);
+        if (lane == 0) {
+            const uint32_t at = (uint32_t)k * TILE + (tid & ~63u);
+            sh.pos_incl[par][k * NW + wave] = im ? at + cut : kWin;
+            sh.pos_empty[par][k * NW + wave] = em ? at + (uint32_t)__builtin_ctzll(em) : kWin;
+            sh.sum[par][k * NW + wave] = part;
         }
-        uint64_t mine = 0;
-#pragma unroll
-        for (int k = 0; k < kLbPer; k++) mine += ((uint32_t)k * TILE + tid <= j) ? (w[k] & kValMask) : 0ull;
-        const uint64_t part = wave_sum_u64(mine);
-        if (lane == 0) sh.sum[wave] = part;
-        __syncthreads();
-        uint64_t total = 0;
-#pragma unroll
-        for (int v = 0; v < TILE / 64; v++) total += sh.sum[v];
-        __syncthreads();
-        excl += total;
-        if (j < kWin) return excl;
-        base -= kWin;
     }
+    if (tid == 0)
+        sh.err[par] = poll ? __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    __syncthreads();
+    uint32_t j = kWin, e = kWin;
+    for (int v = 0; v < P * NW; v++) {
+        const uint32_t x = sh.pos_incl[par][v], y = sh.pos_empty[par][v];
+        j = x < j ? x : j;
+        e = y < e ? y : e;
+    }
+    *err = sh.err[par];
+    // an empty predecessor nearer than the nearest inclusive one (or anywhere,
+    // when the window holds no inclusive one) must be read again
+    if (e < j) return LB_RETRY;
+    // slices are in window order (slice v covers positions 64v .. 64v+63); the
+    // slices before j's hold only aggregates, j's slice sums up to j
+    uint64_t total = 0;
+    for (int v = 0; v < P * NW; v++)
+        if (64u * (uint32_t)v <= j) total += sh.sum[par][v];
+    excl += total;
+    return j < kWin ? LB_DONE : LB_NEXT;
+}
+
+// Look-back from the window ending at tile `base` (inclusive) towards the
+// item's first tile, with `excl` already accumulated and `spins` probes spent:
+// returns the exclusive prefix (in words).  Bounded wait: after kSpinLimit
+// re-reads, or once any tile of the launch has given up, it stops waiting and
+// flags the launch (ERRF_TIMEOUT).  Called by every thread; barriers inside.
+template <int TILE>
+__device__ uint64_t lookback_from(uint64_t *status, int64_t base, uint32_t first, uint32_t epoch,
+                                  LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, uint64_t excl,
+                                  uint32_t spins, uint32_t par)
+{
+    for (;;) {
+        uint64_t w[lb_per(TILE)];
+        lb_probe<TILE>(status, base, first, epoch, w);
+        // the launch-wide error word is polled only while waiting, every 64
+        // probes: it must not become a hot spot for every look-back
+        uint32_t err;
+        const uint32_t r = lb_eval<TILE>(w, epoch, sh, ctrl, (spins & 63u) == 63u, par, excl, &err);
+        par ^= 1u;
+        if (r == LB_DONE) return excl;
+        if (r == LB_NEXT) {
+            base -= (int64_t)lb_per(TILE) * TILE;
+            continue;
+        }
+        if ((err & ERRF_TIMEOUT) || ++spins > kSpinLimit) {
+            if (threadIdx.x == 0)
+                __hip_atomic_fetch_or(&ctrl->error, ERRF_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return 0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Exclusive prefix (in words) of tile `tile` over the tiles [first, tile) of
+// its item by decoupled look-back.
+template <int TILE>
+__device__ __forceinline__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t first,
+                                                  uint32_t epoch, LbShared<TILE / 64 * lb_per(TILE)> &sh,
+                                                  Ctrl *ctrl)
+{
+    return lookback_from<TILE>(status, (int64_t)tile - 1, first, epoch, sh, ctrl, 0, 0, 0);
 }
 
 __device__ __forceinline__ void block_coords(const Item &it, uint32_t b, uint32_t &bx,
@@ -424,19 +437,24 @@ __device__ __forceinline__ const Item &item_at(const Item &single, const Item *i
 
 // Two-stage pipeline per workgroup.  Iteration k holds tile c (packed in LDS,
 // aggregate published, look-back pending) and takes on tile n:
-//   loads of n + first look-back probe of c  (in flight together)
-//   header/scan of n, publish n's aggregate  (n never waits for other tiles)
-//   finish c's look-back, publish c's inclusive prefix, take the next id
-//   store c's headers and payload from LDS, then pack n into LDS.
+//   1. loads of n + the first look-back probe of c   (in flight together)
+//   2. header/scan of n, publish n's aggregate       (n never waits for others)
+//   3. evaluate c's probe; if it resolves, publish c's inclusive prefix and
+//      take the next tile id at once
+//   4. pack n into LDS behind c's payload (a ring: the payload staging area
+//      holds both while their sizes fit, which is the common case), so the
+//      packing overlaps the wait for c's predecessors
+//   5. finish c's look-back if still open, publish, take the next id; store c's
+//      headers and payload; pack n now if it did not fit before.
 // A tile id is taken only after this workgroup's look-back waits are over, so
 // between taking an id and publishing its aggregate a workgroup does only its
 // own work: no look-back ever waits on a reserved, unstarted tile.
 template <int TILE, bool kTable>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(const Item single, const Item *__restrict__ items,
-                                                        const uint32_t *__restrict__ tile_items,
-                                                        uint32_t total_tiles, const uint32_t *d_total_tiles,
-                                                        Ctrl *ctrl, uint64_t *status)
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(
+    const Item single, const Item *__restrict__ items, const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
+    const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
 {
+    constexpr uint32_t kRing = (uint32_t)TILE * 32u;   // staging words: one worst-case tile
     __shared__ TileShared<TILE> sh;
     const uint32_t tid = threadIdx.x;
     if (tid == 0) {
@@ -448,13 +466,13 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
     bool have_cur = false;
-    uint32_t c_tile = 0, c_tl = 0, c_item = 0, c_agg = 0, hb = 0;
+    uint32_t c_tile = 0, c_tl = 0, c_item = 0, c_agg = 0, c_start = 0, hb = 0;
     for (;;) {
         const uint32_t n = sh.next;
         const bool have_n = n < ntot;
         if (!have_n && !have_cur) break;
 
-        // stage 1: loads of n, first probe of c
+        // 1. loads of n, first probe of c
         uint32_t D[32];
         uint32_t n_item = 0, n_tl = 0, nb = 0;
         bool nvalid = false;
@@ -470,11 +488,11 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                 load_block(nit, bx, by, bz, D);
             }
         }
-        uint64_t w0[kLbPer];
-        if (have_cur && c_tl != 0)
-            lb_probe<TILE>(status, (int64_t)c_tile - 1, item_at<kTable>(single, items, c_item).first_tile, epoch, w0);
+        const bool c_lb = have_cur && c_tl != 0;   // c needs a look-back
+        uint64_t w0[lb_per(TILE)];
+        if (c_lb) lb_probe<TILE>(status, (int64_t)c_tile - 1, item_at<kTable>(single, items, c_item).first_tile, epoch, w0);
 
-        // stage 2: header, scan and aggregate of n
+        // 2. header, scan and aggregate of n
         uint32_t n_hdr = 0, n_words = 0, n_off = 0, n_agg = 0;
         if (have_n) {
             const Item &nit = item_at<kTable>(single, items, n_item);
@@ -487,18 +505,50 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (tid == 0) st_relaxed(status + n, lb_word(epoch, n_tl == 0 ? LB_INCL : LB_AGG, n_agg));
         }
 
-        // stage 3: finish c
+        // 3. c's first probe round
+        uint64_t c_excl = 0;
+        uint32_t lb_state = LB_DONE;
+        if (c_lb) {
+            uint32_t err;
+            lb_state = lb_eval<TILE>(w0, epoch, sh.lb, ctrl, false, 0, c_excl, &err);
+            if (lb_state == LB_DONE && tid == 0)
+                st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, c_excl + c_agg));
+        }
+        if (lb_state == LB_DONE && tid == 0)   // no waits remain in this iteration: take the next id
+            sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+        // 4. pack n behind c's payload if both fit in the staging ring
+        uint32_t n_start = 0;
+        bool packed = false;
+        if (have_n) {
+            if (!have_cur) {
+                packed = true;
+            } else if (c_start + c_agg + n_agg <= kRing) {
+                n_start = c_start + c_agg;
+                packed = true;
+            } else if (n_agg <= c_start) {
+                packed = true;
+            }
+            if (packed && n_words)
+                encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
+        }
+
+        // 5. finish c
         if (have_cur) {
             const Item &cit = item_at<kTable>(single, items, c_item);
-            uint64_t excl = 0;
-            if (c_tl != 0) {
-                excl = lookback_tile<TILE>(status, c_tile, cit.first_tile, epoch, sh.lb, ctrl, true, w0);
-                if (tid == 0) st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, excl + c_agg));
+            if (lb_state != LB_DONE) {
+                const int64_t base =
+                    (int64_t)c_tile - 1 - (lb_state == LB_NEXT ? (int64_t)lb_per(TILE) * TILE : 0);
+                c_excl = lookback_from<TILE>(status, base, cit.first_tile, epoch, sh.lb, ctrl, c_excl,
+                                             lb_state == LB_RETRY ? 1u : 0u, 1u);
+                if (tid == 0) {
+                    st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, c_excl + c_agg));
+                    sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
-            if (tid == 0)   // no waits remain in this iteration: take the next id
-                sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t *chdr = sh.hdr[hb ^ 1u];
-            const uint64_t g0 = (uint64_t)cit.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
+            const uint64_t g0 = (uint64_t)cit.hdr_bytes + 4ull * c_excl;   // byte offset of the tile payload
             // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0):
             // 16-byte stores, the last partial chunk by dwords
             if (tid < (uint32_t)TILE / 16) {
@@ -512,26 +562,25 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                         if (o + 4 * k < cit.hdr_bytes) *(uint32_t *)(cit.s_out + o + 4 * k) = chdr[4 * tid + k];
                 }
             }
-            if (g0 + 4ull * c_agg <= cit.s_len) store_payload<TILE>(sh.pay, c_agg, cit.s_out + g0, tid);
+            if (g0 + 4ull * c_agg <= cit.s_len) store_payload<TILE>(sh.pay + c_start, c_agg, cit.s_out + g0, tid);
             if (tid == 0 && c_tl == cit.ntiles - 1 && cit.result) *cit.result = g0 + 4ull * c_agg;
-        } else if (tid == 0) {
-            sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();   // c's payload has left LDS; sh.next is visible
-
-        // stage 4: pack n; it becomes c
-        if (have_n) {
+        __syncthreads();   // c's payload has left LDS; n's packed payload and sh.next are visible
+        if (have_n && !packed) {   // n did not fit beside c: pack it now, at the start of the ring
             if (n_words) encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_off]);
+            __syncthreads();
+        }
+        if (have_n) {
             have_cur = true;
             c_tile = n;
             c_tl = n_tl;
             c_item = n_item;
             c_agg = n_agg;
+            c_start = n_start;
             hb ^= 1u;
         } else {
             have_cur = false;
         }
-        __syncthreads();   // n's payload is in LDS before the next iteration stores it
     }
     finish_launch(ctrl, epoch);
 }
@@ -589,7 +638,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
             excl = 0;
             if (tl != 0) {
-                excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl, false, nullptr);
+                excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
                 if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
             }
         }
@@ -642,7 +691,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // ---------------------------------------------------------------------------
 struct OffShared {
     uint32_t tagg[64];
-    LbShared<4> lb;
+    LbShared<4 * lb_per(256)> lb;
     uint32_t tile, next, epoch, ntot;
     uint32_t excl_lo, excl_hi;
 };
@@ -706,7 +755,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
         const uint32_t cagg = sh.excl_lo;
         uint64_t excl = 0;
         if (chunk != 0) {
-            excl = lookback_tile<256>(status, chunk, 0, epoch, sh.lb, ctrl, false, nullptr);
+            excl = lookback_tile<256>(status, chunk, 0, epoch, sh.lb, ctrl);
             if (tid == 0) st_relaxed(status + chunk, lb_word(epoch, LB_INCL, excl + cagg));
         }
         if (tid < 64) {
