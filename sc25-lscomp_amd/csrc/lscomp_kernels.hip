@@ -8,12 +8,13 @@
 //   2. each lane loads its block once (8 16-byte row loads, coalesced across
 //      lanes: lanes are consecutive blocks along z), computes the header byte;
 //   3. workgroup scan of payload sizes, tile aggregate published at once;
-//   4. two tiles in flight per workgroup: the loads of the next tile overlap
-//      the look-back of the current one, whose payload waits packed in LDS;
-//   5. decoupled look-back with a window of about 1024 predecessors per
-//      probe round (1024/TILE per thread), one 64-bit word per tile (epoch |
-//      flag | value) read and written with relaxed agent-scope atomics, so no
-//      fences are needed on the multi-XCD part;
+//   4. two tiles in flight per workgroup: the loads and packing of the next
+//      tile overlap the look-back of the current one, whose payload waits
+//      packed in LDS;
+//   5. decoupled look-back with a window of 512 predecessors per probe round
+//      (512/TILE per thread), one 64-bit word per tile (epoch | flag | value)
+//      read and written with relaxed agent-scope atomics, so no fences are
+//      needed on the multi-XCD part;
 //   6. headers and payload leave LDS as contiguous coalesced 16-byte stores.
 // Decompression (replaces cuLSZ_decompression_kernel_uint16_bsize64,
 // cuLSZ_kernel.cu:1286-1646) runs the same scan over header sizes, stages the
@@ -231,11 +232,11 @@ __device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len
 }
 
 // Look-back window: each thread reads lb_per(TILE) predecessor words per
-// probe round, so one round covers about 1024 tiles.  The nearest inclusive
+// probe round, so one round covers at least 512 tiles.  The nearest inclusive
 // prefix typically lies about one tile per resident workgroup back, so the
 // first, pre-issued round usually resolves; a larger window costs uncached
-// status reads on every tile.
-constexpr int lb_per(int tile) { return tile >= 1024 ? 1 : (1024 / tile > 4 ? 4 : 1024 / tile); }
+// status reads on every tile (timed: 512 beats 1024 and 2048 at tile 512).
+constexpr int lb_per(int tile) { return tile >= 512 ? 1 : (512 / tile > 4 ? 4 : 512 / tile); }
 
 // LDS of the workgroup-wide look-back, per slice of 64 window positions (one
 // wave's k-th words): nearest inclusive and empty positions and the sum of the
@@ -332,7 +333,7 @@ __device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint
 // re-reads, or once any tile of the launch has given up, it stops waiting and
 // flags the launch (ERRF_TIMEOUT).  Called by every thread; barriers inside.
 template <int TILE>
-__device__ uint64_t lookback_from(uint64_t *status, int64_t base, uint32_t first, uint32_t epoch,
+__device__ __forceinline__ uint64_t lookback_from(uint64_t *status, int64_t base, uint32_t first, uint32_t epoch,
                                   LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, uint64_t excl,
                                   uint32_t spins, uint32_t par)
 {
@@ -393,9 +394,9 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
 }
 
 // Dynamic tile ids.  first_tile() takes an id for immediate processing.  Only
-// where no tile waits for another (decode with precomputed offsets) is the next
-// id taken early with prefetch_tile() and made current with advance_tile(): a
-// reserved tile must never hold up a look-back.
+// where no tile waits for another (decode with precomputed offsets) are ids
+// taken ahead (prefetch_tile() and the decode loop): a reserved tile must never
+// hold up a look-back.
 template <class SH>
 __device__ __forceinline__ void first_tile(SH &sh, Ctrl *ctrl)
 {
@@ -408,12 +409,6 @@ __device__ __forceinline__ void prefetch_tile(SH &sh, Ctrl *ctrl)
 {
     if (threadIdx.x == 0)
         sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <class SH>
-__device__ __forceinline__ void advance_tile(SH &sh)
-{
-    if (threadIdx.x == 0) sh.tile = sh.next;
 }
 
 // Item of a global tile: the single item, or items[tile_item[tile]].
@@ -437,18 +432,21 @@ __device__ __forceinline__ const Item &item_at(const Item &single, const Item *i
 
 // Two-stage pipeline per workgroup.  Iteration k holds tile c (packed in LDS,
 // aggregate published, look-back pending) and takes on tile n:
-//   1. loads of n + the first look-back probe of c   (in flight together)
-//   2. header/scan of n, publish n's aggregate       (n never waits for others)
-//   3. evaluate c's probe; if it resolves, publish c's inclusive prefix and
-//      take the next tile id at once
-//   4. pack n into LDS behind c's payload (a ring: the payload staging area
-//      holds both while their sizes fit, which is the common case), so the
-//      packing overlaps the wait for c's predecessors
-//   5. finish c's look-back if still open, publish, take the next id; store c's
-//      headers and payload; pack n now if it did not fit before.
-// A tile id is taken only after this workgroup's look-back waits are over, so
-// between taking an id and publishing its aggregate a workgroup does only its
-// own work: no look-back ever waits on a reserved, unstarted tile.
+//   1. issue the loads of n and the first look-back probe of c, then take the
+//      next tile id (kept in a register of thread 0 until the end of the
+//      iteration, so the atomic's round trip overlaps all of it)
+//   2. header/scan of n, publish n's aggregate
+//   3. pack n into LDS behind c's payload (a ring: the payload staging area
+//      holds both while their sizes fit, the common case), which gives c's
+//      probe time to arrive
+//   4. evaluate c's probe (re-reading while predecessors are still empty),
+//      publish c's inclusive prefix
+//   5. store c's headers and payload; pack n now if it did not fit before.
+// Forward progress: a tile's look-back waits only for lower tile ids, whose
+// owners are running workgroups (ids are taken dynamically), so the lowest
+// unfinished tile can always proceed.  A reserved id (step 1) waits at most
+// for the rest of the iteration before its loads start; timings showed that
+// hiding the atomic is worth more than the occasional longer wait on c.
 template <int TILE, bool kTable>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(
     const Item single, const Item *__restrict__ items, const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
@@ -472,7 +470,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         const bool have_n = n < ntot;
         if (!have_n && !have_cur) break;
 
-        // 1. loads of n, first probe of c
+        // 1. loads of n, first probe of c, next id
         uint32_t D[32];
         uint32_t n_item = 0, n_tl = 0, nb = 0;
         bool nvalid = false;
@@ -491,6 +489,11 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         const bool c_lb = have_cur && c_tl != 0;   // c needs a look-back
         uint64_t w0[lb_per(TILE)];
         if (c_lb) lb_probe<TILE>(status, (int64_t)c_tile - 1, item_at<kTable>(single, items, c_item).first_tile, epoch, w0);
+        // Issued last: returns are counted in order, so waiting for the loads or
+        // the probe must not include the atomic's longer round trip.
+        uint32_t next_id = 0;
+        if (tid == 0)
+            next_id = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
         // 2. header, scan and aggregate of n
         uint32_t n_hdr = 0, n_words = 0, n_off = 0, n_agg = 0;
@@ -505,19 +508,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (tid == 0) st_relaxed(status + n, lb_word(epoch, n_tl == 0 ? LB_INCL : LB_AGG, n_agg));
         }
 
-        // 3. c's first probe round
-        uint64_t c_excl = 0;
-        uint32_t lb_state = LB_DONE;
-        if (c_lb) {
-            uint32_t err;
-            lb_state = lb_eval<TILE>(w0, epoch, sh.lb, ctrl, false, 0, c_excl, &err);
-            if (lb_state == LB_DONE && tid == 0)
-                st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, c_excl + c_agg));
-        }
-        if (lb_state == LB_DONE && tid == 0)   // no waits remain in this iteration: take the next id
-            sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
-        // 4. pack n behind c's payload if both fit in the staging ring
+        // 3. pack n behind c's payload if both fit in the staging ring
         uint32_t n_start = 0;
         bool packed = false;
         if (have_n) {
@@ -533,19 +524,19 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                 encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
         }
 
-        // 5. finish c
+        // 4-5. finish c
         if (have_cur) {
             const Item &cit = item_at<kTable>(single, items, c_item);
-            if (lb_state != LB_DONE) {
-                const int64_t base =
-                    (int64_t)c_tile - 1 - (lb_state == LB_NEXT ? (int64_t)lb_per(TILE) * TILE : 0);
-                c_excl = lookback_from<TILE>(status, base, cit.first_tile, epoch, sh.lb, ctrl, c_excl,
-                                             lb_state == LB_RETRY ? 1u : 0u, 1u);
-                if (tid == 0) {
-                    st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, c_excl + c_agg));
-                    sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t c_excl = 0;
+            if (c_lb) {
+                uint32_t err;
+                const uint32_t r = lb_eval<TILE>(w0, epoch, sh.lb, ctrl, false, 0, c_excl, &err);
+                if (r != LB_DONE) {
+                    const int64_t base = (int64_t)c_tile - 1 - (r == LB_NEXT ? (int64_t)lb_per(TILE) * TILE : 0);
+                    c_excl = lookback_from<TILE>(status, base, cit.first_tile, epoch, sh.lb, ctrl, c_excl,
+                                                 r == LB_RETRY ? 1u : 0u, 1u);
                 }
+                if (tid == 0) st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, c_excl + c_agg));
             }
             const uint32_t *chdr = sh.hdr[hb ^ 1u];
             const uint64_t g0 = (uint64_t)cit.hdr_bytes + 4ull * c_excl;   // byte offset of the tile payload
@@ -565,6 +556,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (g0 + 4ull * c_agg <= cit.s_len) store_payload<TILE>(sh.pay + c_start, c_agg, cit.s_out + g0, tid);
             if (tid == 0 && c_tl == cit.ntiles - 1 && cit.result) *cit.result = g0 + 4ull * c_agg;
         }
+        if (tid == 0) sh.next = next_id;
         __syncthreads();   // c's payload has left LDS; n's packed payload and sh.next are visible
         if (have_n && !packed) {   // n did not fit beside c: pack it now, at the start of the ring
             if (n_words) encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_off]);
@@ -602,13 +594,16 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
     }
     first_tile(sh, ctrl);
+    if (tile_off) prefetch_tile(sh, ctrl);
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
     // With precomputed tile offsets (no look-back) the header byte and the
-    // offset of the next tile are loaded while the current tile decodes; the
-    // next tile id is then taken early, which is safe because no tile waits
-    // for another.  With the look-back, ids are taken one at a time.
+    // offset of the next tile are loaded while the current tile decodes, and
+    // tile ids are taken two ahead (the id taken in an iteration is used two
+    // iterations later, so the atomic's round trip never stalls a wave); that
+    // is safe because no tile waits for another.  With the look-back, ids are
+    // taken one at a time.
     uint32_t hdr = 0;
     uint64_t excl_next = 0;
     if (tile_off && sh.tile < ntot) {
@@ -621,7 +616,10 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
-        if (tile_off) prefetch_tile(sh, ctrl);
+        const uint32_t ntile = tile_off ? sh.next : ntot;
+        uint32_t after_next = 0;   // thread 0: the id two tiles ahead
+        if (tile_off && tid == 0)
+            after_next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
         const uint32_t b = tl * (uint32_t)TILE + tid;
@@ -633,7 +631,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         bool bad = (hdr & 0x1fu) > 16u;
 
         uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);   // barrier: sh.next is visible
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
         if (!tile_off) {
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
             excl = 0;
@@ -649,7 +647,6 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
         // next tile's header byte and offset
         const uint32_t hdr_cur = hdr;
-        const uint32_t ntile = tile_off ? sh.next : ntot;
         if (ntile < ntot) {
             const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
             const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
@@ -674,8 +671,14 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                                            __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        if (tile_off) advance_tile(sh);
-        else first_tile(sh, ctrl);
+        if (tile_off) {
+            if (tid == 0) {
+                sh.tile = ntile;
+                sh.next = after_next;
+            }
+        } else {
+            first_tile(sh, ctrl);
+        }
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
