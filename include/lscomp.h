@@ -238,6 +238,13 @@ typedef struct lscomp_error_stats {
 int lscomp_error_stats_u16(const uint16_t *d_orig, const uint16_t *d_dec, size_t n, lscomp_error_stats *out,
                            hipStream_t stream);
 
+/* Measurement helper: a plain device copy of `bytes` bytes (d_dst, d_src
+ * 16-byte aligned, non-overlapping), one 16-byte load and store per lane like
+ * the codec kernels' row accesses.  Gives the copy bandwidth and the scale of
+ * the FETCH_SIZE / WRITE_SIZE counters for such accesses.  No counterpart in
+ * the reference. */
+int lscomp_copy_bytes(void *d_dst, const void *d_src, size_t bytes, hipStream_t stream);
+
 const char *lscomp_status_string(int status);
 
 /* Library version, major*10000 + minor*100 + patch. */

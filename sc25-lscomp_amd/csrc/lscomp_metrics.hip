@@ -6,7 +6,8 @@
 // sqrt(sum dec^2), nrmse = sqrt(mse)/range; plus the maximum absolute error
 // and the maximum pointwise relative error over nonzero originals.  Sums are
 // exact 64-bit integers (uint16 data), reduced per wave and then atomically.
-// One pass, HBM-bound: 4 bytes read per element pair.
+// One pass, HBM-bound: 4 bytes read per element pair.  Also the plain copy
+// used to measure copy bandwidth and to calibrate the HBM byte counters.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -94,7 +95,50 @@ __global__ __launch_bounds__(256) void metrics_kernel(const uint16_t *__restrict
     }
 }
 
+// Plain copy: grid-stride 16-byte loads and stores, four in flight per lane.
+__global__ __launch_bounds__(256) void copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
+                                                   uint64_t n16)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+__global__ void copy_tail_kernel(uint8_t *dst, const uint8_t *src, uint32_t n)
+{
+    if (threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
+}
+
 }  // namespace lsc
+
+extern "C" int lscomp_copy_bytes(void *d_dst, const void *d_src, size_t bytes, hipStream_t stream)
+{
+    using namespace lsc;
+    if (bytes == 0) return LSCOMP_OK;
+    if (!d_dst || !d_src || (((uintptr_t)d_dst | (uintptr_t)d_src) & 15)) return LSCOMP_ERR_INVALID_ARG;
+    const uint8_t *s = (const uint8_t *)d_src;
+    uint8_t *d = (uint8_t *)d_dst;
+    if (d < s + bytes && s < d + bytes) return LSCOMP_ERR_INVALID_ARG;   // overlap
+    const uint64_t n16 = bytes / 16;
+    if (n16) {
+        uint64_t blocks = (n16 + 255) / 256;
+        const uint64_t cap = (uint64_t)num_cus() * 16;
+        if (blocks > cap) blocks = cap;
+        hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, (uint4 *)d,
+                           (const uint4 *)s, n16);
+    }
+    const uint32_t tail = (uint32_t)(bytes - 16 * n16);
+    if (tail)
+        hipLaunchKernelGGL(copy_tail_kernel, dim3(1), dim3(16), 0, stream, d + 16 * n16, s + 16 * n16, tail);
+    return hip_fail(hipGetLastError(), "copy launch");
+}
 
 extern "C" int lscomp_error_stats_u16(const uint16_t *d_orig, const uint16_t *d_dec, size_t n,
                                       lscomp_error_stats *out, hipStream_t stream)

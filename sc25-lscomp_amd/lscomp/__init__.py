@@ -64,7 +64,7 @@ EXPORTS = (
     "lscomp_plan_launch_graph", "lscomp_plan_device_sizes", "lscomp_plan_sizes", "lscomp_plan_status",
     "lscomp_plan_destroy", "lscomp_server_create", "lscomp_server_submit_u16", "lscomp_server_poll",
     "lscomp_server_release", "lscomp_server_drain", "lscomp_server_replay_u16", "lscomp_server_destroy",
-    "lscomp_autotune_u16", "lscomp_set_decode_prepass", "lscomp_error_stats_u16",
+    "lscomp_autotune_u16", "lscomp_set_decode_prepass", "lscomp_error_stats_u16", "lscomp_copy_bytes",
 )
 
 
@@ -156,6 +156,8 @@ def _load():
     lib.lscomp_server_destroy.argtypes = [vp]
     lib.lscomp_error_stats_u16.restype = ctypes.c_int
     lib.lscomp_error_stats_u16.argtypes = [vp, vp, sz, ctypes.POINTER(ErrorStats), vp]
+    lib.lscomp_copy_bytes.restype = ctypes.c_int
+    lib.lscomp_copy_bytes.argtypes = [vp, vp, sz, vp]
     lib.lscomp_set_decode_prepass.restype = ctypes.c_int
     lib.lscomp_set_decode_prepass.argtypes = [ctypes.c_int]
     lib.lscomp_autotune_u16.restype = ctypes.c_int
@@ -330,6 +332,18 @@ def error_stats(orig, dec, stream=None) -> dict:
     if st != OK:
         raise LscompError(st, "lscomp_error_stats_u16")
     return {name: getattr(out, name) for name, _ in ErrorStats._fields_ if name != "reserved"}
+
+
+def copy_bytes(dst, src, stream=None) -> None:
+    """Plain device copy of src into dst (same byte size, 16-byte aligned): the
+    reference copy for bandwidth and HBM counter calibration."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nbytes:
+        raise ValueError("dst and src must have the same size in bytes")
+    st = _lib.lscomp_copy_bytes(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nbytes,
+                                _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_copy_bytes")
 
 
 def set_decode_prepass(enable: bool) -> bool:

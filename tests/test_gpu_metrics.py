@@ -42,3 +42,21 @@ def test_metrics_of_a_lossy_round_trip(gpu):
     out, n = gpu.compress(d, vol.shape, (3, 5, 10, 15), 1.0)        # no pooling: error < bin
     s = gpu.error_stats(d, gpu.decompress(out, n, vol.shape, (3, 5, 10, 15)))
     assert 0 < s["max_abs_err"] <= 14 and np.isfinite(s["psnr"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbytes", [16, 1000, 4096 + 7, 3 * 1024 * 1024 + 5])
+def test_copy_bytes(gpu, nbytes):
+    """The calibration copy reproduces its source exactly, tail bytes included,
+    and leaves the bytes after the destination alone."""
+    import torch
+    src = torch.randint(0, 256, (nbytes,), dtype=torch.uint8, device="cuda")
+    dst = torch.full((nbytes + 32,), 7, dtype=torch.uint8, device="cuda")
+    gpu.copy_bytes(dst[:nbytes], src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:nbytes], src)
+    assert bool((dst[nbytes:] == 7).all())
+    with pytest.raises(ValueError):
+        gpu.copy_bytes(dst[:nbytes - 1], src)
+    with pytest.raises(gpu.LscompError):
+        gpu.copy_bytes(dst[1:nbytes + 1], src)   # misaligned destination
