@@ -95,20 +95,19 @@ __global__ __launch_bounds__(256) void metrics_kernel(const uint16_t *__restrict
     }
 }
 
-// Plain copy: grid-stride 16-byte loads and stores, four in flight per lane.
+// Plain copy: each workgroup copies one contiguous 16 KiB chunk, four 16-byte
+// loads per lane issued before the four stores.
 __global__ __launch_bounds__(256) void copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
                                                    uint64_t n16)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n16; i += 4 * stride) {
-        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
-    }
-    for (; i < n16; i += stride) dst[i] = src[i];
+    const uint64_t base = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (base + 256 * k < n16) v[k] = src[base + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (base + 256 * k < n16) dst[base + 256 * k] = v[k];
 }
 
 __global__ void copy_tail_kernel(uint8_t *dst, const uint8_t *src, uint32_t n)
@@ -128,9 +127,8 @@ extern "C" int lscomp_copy_bytes(void *d_dst, const void *d_src, size_t bytes, h
     if (d < s + bytes && s < d + bytes) return LSCOMP_ERR_INVALID_ARG;   // overlap
     const uint64_t n16 = bytes / 16;
     if (n16) {
-        uint64_t blocks = (n16 + 255) / 256;
-        const uint64_t cap = (uint64_t)num_cus() * 16;
-        if (blocks > cap) blocks = cap;
+        const uint64_t blocks = (n16 + 1023) / 1024;   // 16 KiB per workgroup
+        if (blocks > 0xffffffffull) return LSCOMP_ERR_INVALID_ARG;
         hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, (uint4 *)d,
                            (const uint4 *)s, n16);
     }
