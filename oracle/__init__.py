@@ -1,8 +1,9 @@
-"""CPU oracle of the lsCOMP uint16 codec -- TEST INFRASTRUCTURE ONLY.
+"""CPU oracle of the lsCOMP uint16 and uint32 codecs -- TEST INFRASTRUCTURE ONLY.
 
 ctypes wrapper over ``liblscomp_oracle.so`` (built from ``lscomp_oracle.c`` by
 ``make -C oracle``), a scalar C restatement of the reference's per-block logic
-(lsCOMP/src/cuLSZ_kernel.cu:827-1646, lsCOMP/src/cuLSZ_entry.cu:93-166).
+(lsCOMP/src/cuLSZ_kernel.cu:827-1646, lsCOMP/src/cuLSZ_entry.cu:93-166; uint32:
+cuLSZ_kernel.cu:4-824, cuLSZ_entry.cu:18-91).
 Only the tests, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may use it, and only as the checker or the timed CPU baseline;
 the product library never calls it.
@@ -17,7 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liblscomp_oracle.so")
 
-OK, ERR_ARG, ERR_CAPACITY, ERR_CORRUPT = 0, 1, 2, 3
+OK, ERR_ARG, ERR_CAPACITY, ERR_CORRUPT, ERR_RANGE = 0, 1, 2, 3, 4
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u16p = ctypes.POINTER(ctypes.c_uint16)
@@ -70,6 +71,12 @@ class Oracle:
             if name.endswith("_omp"):
                 args.append(_u64p)
             f.argtypes = args
+        lib.oracle_max_compressed_bytes_u32.restype = _u64
+        lib.oracle_max_compressed_bytes_u32.argtypes = [_u32, _u32, _u32]
+        lib.oracle_compress_u32.restype = ctypes.c_int
+        lib.oracle_compress_u32.argtypes = [_u32p, _u32, _u32, _u32, _u32p, ctypes.c_float, _u8p, _u64, _u64p]
+        lib.oracle_decompress_u32.restype = ctypes.c_int
+        lib.oracle_decompress_u32.argtypes = [_u8p, _u64, _u32, _u32, _u32, _u32p, _u32p]
         lib.oracle_max_threads.restype = ctypes.c_int
         self.lib = lib
 
@@ -141,6 +148,34 @@ class Oracle:
             st = self.lib.oracle_decompress_u16_omp(*args, scratch.ctypes.data_as(_u64p))
         else:
             st = self.lib.oracle_decompress_u16(*args)
+        return int(st), out
+
+    # -- uint32 volumes ----------------------------------------------------
+    def max_compressed_bytes_u32(self, dims) -> int:
+        return int(self.lib.oracle_max_compressed_bytes_u32(*map(int, dims)))
+
+    def compress_u32(self, vol: np.ndarray, bins, p: float, capacity: int | None = None):
+        """Returns (status, stream bytes as np.uint8 array, required size)."""
+        vol = np.ascontiguousarray(vol, dtype=np.uint32)
+        x, y, z = vol.shape
+        cap = self.max_compressed_bytes_u32((x, y, z)) if capacity is None else capacity
+        out = np.zeros(max(cap, 1), np.uint8)
+        b = np.ascontiguousarray(bins, dtype=np.uint32)
+        n = _u64(0)
+        st = self.lib.oracle_compress_u32(vol.ctypes.data_as(_u32p), x, y, z, b.ctypes.data_as(_u32p),
+                                          ctypes.c_float(p), out.ctypes.data_as(_u8p), cap, ctypes.byref(n))
+        size = int(n.value)
+        return int(st), out[: min(size, cap)], size
+
+    def decompress_u32(self, stream, dims, bins, fill: int = 0):
+        """Returns (status, volume uint32 (x, y, z))."""
+        s = np.ascontiguousarray(np.frombuffer(bytes(stream), np.uint8) if not isinstance(stream, np.ndarray)
+                                 else stream, dtype=np.uint8)
+        x, y, z = map(int, dims)
+        out = np.full((x, y, z), fill, np.uint32)
+        b = np.ascontiguousarray(bins, dtype=np.uint32)
+        st = self.lib.oracle_decompress_u32(s.ctypes.data_as(_u8p), s.size, x, y, z, b.ctypes.data_as(_u32p),
+                                            out.ctypes.data_as(_u32p))
         return int(st), out
 
     def max_threads(self) -> int:

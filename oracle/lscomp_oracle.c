@@ -31,6 +31,17 @@
  *     sum(payload) + B, cuLSZ_entry.cu:125, and B alone for grids of one CTA);
  *   - element indices are 64-bit (the reference uses 32-bit, :857, :886).
  *
+ * uint32 volumes (the reference's cuLSZ_*_uint32_bsize64, lsCOMP/src/
+ * cuLSZ_kernel.cu:4-824, lsCOMP/src/cuLSZ_entry.cu:18-91) use the same
+ * per-block functions: the pair average wraps modulo 2^32 like the
+ * reference's uint sum (cuLSZ_kernel.cu:91, :286), de-quantization wraps
+ * modulo 2^32, and rates reach 31.  A block whose quantized maximum needs 32
+ * bits would get rate 32, which sets bit 5 of the header (cuLSZ_kernel.cu:
+ * 120-122, 129-131) and corrupts its bin choice; the reference then writes no
+ * payload but advances the offset by 256 bytes (:259-261 vs :121, :130), so
+ * its stream cannot be decoded.  Here such a volume is rejected
+ * (ORC_ERR_RANGE), and the worst case per block is 31 planes (248 bytes).
+ *
  * The OpenMP variants (oracle_*_omp) produce the same bytes with a two-pass
  * schedule (headers + sizes, exclusive scan, pack) and are what bench.py times
  * as the CPU baseline.
@@ -53,6 +64,7 @@
 #define ORC_ERR_ARG 1
 #define ORC_ERR_CAPACITY 2
 #define ORC_ERR_CORRUPT 3
+#define ORC_ERR_RANGE 4
 
 /* ---- blocking (cuLSZ_entry.cu:97-99, cuLSZ_kernel.cu:843) ---------------- */
 
@@ -115,6 +127,42 @@ static void scatter_block(uint16_t *vol, uint32_t x, uint32_t y, uint32_t z,
             uint64_t zz = bz * 8 + k;
             if (yy < y && zz < z)   /* cuLSZ_kernel.cu:1638 */
                 vol[bx * (uint64_t)y * z + yy * z + zz] = (uint16_t)v[r * 8 + k];
+        }
+    }
+}
+
+/* uint32 volumes: same blocking, 4-byte elements (cuLSZ_kernel.cu:41-72). */
+void oracle_gather_block_u32(const uint32_t *vol, uint32_t x, uint32_t y, uint32_t z,
+                             uint64_t b, uint32_t v[64])
+{
+    uint64_t nby = ((uint64_t)y + 7) / 8, nbz = ((uint64_t)z + 7) / 8;
+    uint64_t per_frame = nby * nbz;
+    uint64_t bx = b / per_frame;
+    uint64_t by = (b % per_frame) / nbz;
+    uint64_t bz = (b % per_frame) % nbz;
+    (void)x;
+    for (int r = 0; r < 8; r++) {
+        uint64_t yy = by * 8 + r;
+        for (int k = 0; k < 8; k++) {
+            uint64_t zz = bz * 8 + k;
+            v[r * 8 + k] = (yy < y && zz < z) ? vol[bx * (uint64_t)y * z + yy * z + zz] : 0;
+        }
+    }
+}
+
+static void scatter_block_u32(uint32_t *vol, uint32_t y, uint32_t z, uint64_t b, const uint32_t v[64])
+{
+    uint64_t nby = ((uint64_t)y + 7) / 8, nbz = ((uint64_t)z + 7) / 8;
+    uint64_t per_frame = nby * nbz;
+    uint64_t bx = b / per_frame;
+    uint64_t by = (b % per_frame) / nbz;
+    uint64_t bz = (b % per_frame) % nbz;
+    for (int r = 0; r < 8; r++) {
+        uint64_t yy = by * 8 + r;
+        for (int k = 0; k < 8; k++) {
+            uint64_t zz = bz * 8 + k;
+            if (yy < y && zz < z)   /* cuLSZ_kernel.cu:815 */
+                vol[bx * (uint64_t)y * z + yy * z + zz] = v[r * 8 + k];
         }
     }
 }
@@ -334,6 +382,69 @@ int oracle_decompress_u16(const uint8_t *in, uint64_t in_bytes,
             oracle_decode_block(h, in + pos, bins, v);
         }
         scatter_block(out, x, y, z, b, v);
+        pos += n;
+    }
+    return bad ? ORC_ERR_CORRUPT : ORC_OK;
+}
+
+/* ---- uint32 volumes (cuLSZ_kernel.cu:4-824, cuLSZ_entry.cu:18-91) -------- */
+
+uint64_t oracle_max_compressed_bytes_u32(uint32_t x, uint32_t y, uint32_t z)
+{
+    uint64_t b = oracle_num_blocks(x, y, z);
+    return oracle_header_bytes(b) + 248 * b;
+}
+
+int oracle_compress_u32(const uint32_t *in, uint32_t x, uint32_t y, uint32_t z,
+                        const uint32_t bins[4], float pooling_th,
+                        uint8_t *out, uint64_t capacity, uint64_t *out_bytes)
+{
+    int st = oracle_check_params(x, y, z, bins, pooling_th);
+    if (st) return st;
+    uint64_t nb = oracle_num_blocks(x, y, z);
+    uint64_t hb = oracle_header_bytes(nb);
+    if (capacity < hb) { *out_bytes = hb; return ORC_ERR_CAPACITY; }
+    uint64_t pos = hb;
+    for (uint64_t i = nb; i < hb; i++) out[i] = 0;
+    uint32_t v[64];
+    uint8_t tmp[256];
+    int over = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint32_t pb;
+        oracle_gather_block_u32(in, x, y, z, b, v);
+        uint8_t h = oracle_block_header(v, bins, pooling_th, &pb);
+        if (pb / ((h >> 7) ? 4u : 8u) > 31) { *out_bytes = 0; return ORC_ERR_RANGE; }   /* rate 32 */
+        out[b] = h;
+        uint32_t n = oracle_encode_block(v, h, bins, tmp);
+        if (!over && pos + n <= capacity) memcpy(out + pos, tmp, n);
+        else over = 1;
+        pos += n;
+    }
+    *out_bytes = pos;
+    return over ? ORC_ERR_CAPACITY : ORC_OK;
+}
+
+int oracle_decompress_u32(const uint8_t *in, uint64_t in_bytes,
+                          uint32_t x, uint32_t y, uint32_t z, const uint32_t bins[4],
+                          uint32_t *out)
+{
+    if (oracle_check_params(x, y, z, bins, 0.0f)) return ORC_ERR_ARG;
+    uint64_t nb = oracle_num_blocks(x, y, z);
+    uint64_t hb = oracle_header_bytes(nb);
+    if (in_bytes < hb) return ORC_ERR_CORRUPT;
+    uint64_t pos = hb;
+    uint32_t v[64];
+    int bad = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint8_t h = in[b];
+        uint32_t n = header_payload_bytes(h);
+        if (pos + n > in_bytes) {
+            bad = 1;
+            memset(v, 0, sizeof v);
+        } else {
+            oracle_decode_block(h, in + pos, bins, v);
+        }
+        scatter_block_u32(out, y, z, b, v);
         pos += n;
     }
     return bad ? ORC_ERR_CORRUPT : ORC_OK;

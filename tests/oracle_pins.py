@@ -16,6 +16,8 @@ every mutant must fail at least one pin.
 """
 import numpy as np
 
+import oracle
+
 LOSSLESS = (1, 1, 1, 1)
 
 
@@ -268,9 +270,93 @@ def pin_omp_equals_serial(orc):
                                   orc.decompress(a[1], vol.shape, bins, omp=True)[1])
 
 
+# ---- uint32 volumes (cuLSZ_kernel.cu:4-824) ---------------------------------
+
+def _u32_block(entries):
+    v = np.zeros((1, 8, 8), np.uint32)
+    for (r, k), val in entries.items():
+        v[0, r, k] = val
+    return v
+
+
+def pin_u32_kat(orc):
+    """Hand-derived uint32 streams: the wrapping pair average, rates above 16 and the rate-32 rejection."""
+    # (0xFFFFFFFF + 3) mod 2^32 = 2 -> average 1; 62 zeros > 0.5*64 -> pooled, all bins 1 -> choice 3, rate 1
+    st, s, n = orc.compress_u32(_u32_block({(0, 0): 0xFFFFFFFF, (0, 1): 3}), LOSSLESS, 0.5)
+    assert st == 0 and bytes(s[:n]) == bytes([0xE1, 0, 0, 0, 0x80, 0, 0, 0])
+    st, dec = orc.decompress_u32(s[:n], (1, 8, 8), LOSSLESS)
+    assert st == 0 and dec[0, 0, 0] == 1 and dec[0, 0, 1] == 1 and not dec.ravel()[2:].any()
+    # rate 20 plain: q[0][0] = 2^20 - 1 sets bit 7 of byte 0 in planes 0..19
+    st, s, n = orc.compress_u32(_u32_block({(0, 0): (1 << 20) - 1}), LOSSLESS, 1.0)
+    assert st == 0 and bytes(s[:n]) == bytes([0x74, 0, 0, 0]) + bytes([0x80, 0, 0, 0, 0, 0, 0, 0]) * 20
+    # rate 31 plain: q[7][7] = 2^31 - 1 sets bit 0 of byte 7 in planes 0..30
+    st, s, n = orc.compress_u32(_u32_block({(7, 7): (1 << 31) - 1}), LOSSLESS, 1.0)
+    assert st == 0 and bytes(s[:n]) == bytes([0x7F, 0, 0, 0]) + bytes([0, 0, 0, 0, 0, 0, 0, 1]) * 31
+    st, dec = orc.decompress_u32(s[:n], (1, 8, 8), LOSSLESS)
+    assert st == 0 and dec[0, 7, 7] == (1 << 31) - 1
+    # a quantized maximum of 2^31 needs rate 32, which the 5-bit field cannot hold: rejected;
+    # with bin 2 the same block has rate 31
+    v = _u32_block({(0, 5): 1 << 31})
+    assert orc.compress_u32(v, LOSSLESS, 1.0)[0] == oracle.ERR_RANGE
+    st, s, n = orc.compress_u32(v, (2, 2, 2, 2), 1.0)
+    assert st == 0 and s[0] == 0x7F and n == 4 + 31 * 8
+
+
+def _volumes_u32():
+    rng = np.random.default_rng(4321)
+    vols = []
+    for shape in [(1, 8, 8), (2, 13, 17), (3, 24, 40), (1, 8, 1), (2, 31, 66)]:
+        u = rng.integers(0, 1 << 31, size=shape, dtype=np.uint64).astype(np.uint32)
+        u[rng.random(shape) < 0.5] = 0
+        vols.append(u)
+        vols.append((rng.poisson(2.0, size=shape) * rng.integers(1, 1 << 20, size=shape)).astype(np.uint32))
+    return vols
+
+
+def pin_u32_structure(orc):
+    """Lossless round trip and the stream layout from numpy-gathered blocks, for uint32 volumes."""
+    for vol in _volumes_u32():
+        st, s, n = orc.compress_u32(vol, LOSSLESS, 1.0)
+        st2, dec = orc.decompress_u32(s[:n], vol.shape, LOSSLESS, fill=0xDEADBEEF)
+        assert st == 0 and st2 == 0 and np.array_equal(dec, vol)
+        for bins, p in ((LOSSLESS, 1.0), ((3, 5, 10, 15), 0.5), ((1000, 1000, 70000, 70000), 0.25)):
+            st, s, n = orc.compress_u32(vol, bins, p)
+            blocks = _blocks_numpy(vol)
+            nb = len(blocks)
+            R = -(-nb // 4) * 4
+            assert st == 0 and orc.max_compressed_bytes_u32(vol.shape) == R + 248 * nb
+            hdrs = np.frombuffer(bytes(s[:nb]), np.uint8)
+            assert all(b == 0 for b in bytes(s[nb:R]))
+            sizes = _sizes(hdrs)
+            assert n == R + int(sizes.sum())
+            offs = R + np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            for b in range(nb):
+                h, nbytes = orc.block_header(blocks[b], bins, p)
+                assert h == hdrs[b] and nbytes == sizes[b]
+                assert bytes(s[offs[b]: offs[b] + sizes[b]]) == orc.encode_block(blocks[b], h, bins)
+            st2, dec = orc.decompress_u32(s[:n], vol.shape, bins)
+            # plain blocks: 0 <= v - v' < bin (no pooling case)
+            if p >= 1.0:
+                d = vol.astype(np.int64) - dec.astype(np.int64)
+                assert st2 == 0 and np.all(d >= 0) and np.all(d < max(bins))
+
+
+def pin_u32_matches_u16(orc):
+    """The reference's uint16 and uint32 kernels differ only in offset types: a uint32 volume holding
+    uint16 values gives the uint16 stream, byte for byte."""
+    for vol in _volumes()[:9]:
+        for bins, p in ((LOSSLESS, 1.0), ((3, 5, 10, 15), 0.5), ((2, 9, 9, 30), 0.0)):
+            a = orc.compress(vol, bins, p)
+            b = orc.compress_u32(vol.astype(np.uint32), bins, p)
+            assert a[0] == b[0] == 0 and a[2] == b[2] and np.array_equal(a[1], b[1])
+            assert np.array_equal(orc.decompress(a[1], vol.shape, bins)[1].astype(np.uint32),
+                                  orc.decompress_u32(b[1], vol.shape, bins)[1])
+
+
 ALL_PINS = [pin_kat_bytes, pin_kat_decode, pin_onehot_plain, pin_onehot_pooled, pin_pool_threshold,
             pin_bin_choice, pin_rate_edges, pin_lossless_roundtrip, pin_stream_structure, pin_lossy_bounds,
-            pin_header_locality, pin_frame_prefix, pin_omp_equals_serial]
+            pin_header_locality, pin_frame_prefix, pin_omp_equals_serial, pin_u32_kat, pin_u32_structure,
+            pin_u32_matches_u16]
 
 # planted mistakes in oracle/lscomp_oracle.c (ORACLE_MUTANT=n)
 MUTANTS = {1: "plain bits LSB-first", 2: "plain block transposed", 3: "pool on >=", 4: "bin counts use <=",
