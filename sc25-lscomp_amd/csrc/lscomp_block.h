@@ -87,6 +87,7 @@ struct Decide {
     float f[4];        // fl((1 + 2^-20) / b[k])
     uint32_t zth;      // pool iff zero_count >= zth (65: never)
     uint32_t flags;    // DEC_*
+    FastDiv fd[4];     // exact 32-bit division by b[k] (uint32 volumes)
 };
 enum : uint32_t {
     DEC_LOSSLESS = 1u,  // all bins 1 and no pooling: q == v
@@ -337,6 +338,163 @@ LSC_HD void decode_block(const uint32_t *in, uint32_t hdr, const uint32_t bins[4
         const u16x2 bb = {(unsigned short)bin, (unsigned short)bin};
 #pragma unroll
         for (int i = 0; i < 32; i++) D[i] = as_u32(as_u16x2(D[i]) * bb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// uint32 blocks (cuLSZ_kernel.cu:4-824): 64 values V[8r + k], same decision
+// and bit layout as uint16, with 32-bit arithmetic: the pair average wraps
+// modulo 2^32 like the reference's uint sum, divisions are exact 32-bit
+// (fdiv), de-quantization wraps modulo 2^32, and rates reach 31.  The planes
+// are built from the values' low and high 16-bit halves packed in pairs, so
+// the uint16 column-word code does the bit transposition.
+// ---------------------------------------------------------------------------
+
+// Header byte of a uint32 block.  *rate32 is set when the quantized maximum
+// needs 32 bits, which the 5-bit rate field cannot hold (the reference then
+// corrupts the bin choice and leaves an unwritten 256-byte hole, so such
+// volumes are rejected); the returned byte is then meaningless.
+LSC_HD uint32_t block_header_u32(const uint32_t V[64], const Decide &p, bool *rate32)
+{
+    uint32_t m1 = 0;
+#pragma unroll
+    for (int i = 0; i < 64; i++) m1 = V[i] > m1 ? V[i] : m1;
+    uint32_t pool = 0, m2 = 0;
+    if (p.flags & DEC_POOL) {
+        uint32_t nz = 0;
+#pragma unroll
+        for (int i = 0; i < 64; i++) nz += V[i] != 0u;
+#pragma unroll
+        for (int t = 0; t < 32; t++) {
+            const uint32_t a = (V[2 * t] + V[2 * t + 1]) >> 1;   // wraps like the reference
+            m2 = a > m2 ? a : m2;
+        }
+        pool = (64u - nz) >= p.zth;
+    }
+    // bin choice: bc = #{k in 1..3 : m0 >= b[k]}, m0 = min{v : v >= b0}
+    // (none: 3), from the minimum of (v - b0) mod 2^32
+    uint32_t bc = 3;
+    if (p.b[0] != p.b[3]) {
+        uint32_t mt = 0xffffffffu;
+#pragma unroll
+        for (int i = 0; i < 64; i++) {
+            const uint32_t t = V[i] - p.b[0];
+            mt = t < mt ? t : mt;
+        }
+        if (mt <= 0xffffffffu - p.b[0]) {
+            const uint32_t m0 = mt + p.b[0];
+            bc = (uint32_t)(m0 >= p.b[1]) + (uint32_t)(m0 >= p.b[2]) + (uint32_t)(m0 >= p.b[3]);
+        }
+    }
+    const uint32_t mx = pool ? m2 : m1;
+    const uint32_t maxq = p.b[bc] == 1u ? mx : fdiv(mx, p.fd[bc]);
+    const uint32_t rate = bit_width(maxq);
+    *rate32 = rate > 31u;
+    return (pool << 7) | (bc << 5) | (rate & 0x1fu);
+}
+
+// Encode the payload of a uint32 block into words out[0..] (2*rate words
+// plain, rate pooled); returns the word count.  Overwrites V.
+LSC_HD uint32_t encode_block_u32(uint32_t V[64], uint32_t hdr, const Decide &p, uint32_t *out)
+{
+    const uint32_t rate = hdr & 0x1fu;
+    if (rate == 0) return 0;
+    const uint32_t bc = (hdr >> 5) & 3u;
+    const bool identity = p.b[bc] == 1u;
+    const FastDiv fd = p.fd[bc];
+    if (hdr & 0x80u) {
+        uint32_t Q[32], C[8];
+#pragma unroll
+        for (int t = 0; t < 32; t++) {
+            const uint32_t a = (V[2 * t] + V[2 * t + 1]) >> 1;
+            Q[t] = identity ? a : fdiv(a, fd);
+        }
+        for (uint32_t g = 0; 8 * g < rate; g++) {   // planes 8g .. 8g+7: byte g of each value
+            if (g == 2) {
+#pragma unroll
+                for (int t = 0; t < 32; t++) Q[t] >>= 16;
+            }
+            pooled_columns(Q, C, (g & 1u) != 0);
+            const uint32_t end = rate < 8 * g + 8 ? rate : 8 * g + 8;
+            for (uint32_t i = 8 * g; i < end; i++) out[i] = plane_from_columns(C, i - 8 * g);
+        }
+        return rate;
+    }
+    if (!identity) {
+#pragma unroll
+        for (int e = 0; e < 64; e++) V[e] = fdiv(V[e], fd);
+    }
+    // D[i] = pair i (elements 2i, 2i+1) of the low (first pass) or high halves
+    uint32_t D[32], C[16];
+#pragma unroll
+    for (int i = 0; i < 32; i++) D[i] = perm(V[2 * i + 1], V[2 * i], 0x05040100u);
+    for (uint32_t g = 0; 8 * g < rate; g++) {
+        if (g == 2) {
+#pragma unroll
+            for (int i = 0; i < 32; i++) D[i] = perm(V[2 * i + 1], V[2 * i], 0x07060302u);
+        }
+        columns_from_rows(D, C, (g & 1u) != 0);
+        const uint32_t end = rate < 8 * g + 8 ? rate : 8 * g + 8;
+        for (uint32_t i = 8 * g; i < end; i++) {
+            out[2 * i] = plane_from_columns(C, i - 8 * g);
+            out[2 * i + 1] = plane_from_columns(C + 8, i - 8 * g);
+        }
+    }
+    return 2 * rate;
+}
+
+// Decode a uint32 block from its payload words.  rate <= 31.
+LSC_HD void decode_block_u32(const uint32_t *in, uint32_t hdr, const uint32_t bins[4], uint32_t V[64])
+{
+    const uint32_t rate = hdr & 0x1fu;
+    const uint32_t bin = bins[(hdr >> 5) & 3u];
+#pragma unroll
+    for (int e = 0; e < 64; e++) V[e] = 0;
+    if (rate == 0) return;
+    if (hdr & 0x80u) {
+        for (uint32_t g = 0; 8 * g < rate; g++) {
+            uint32_t CP[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            const uint32_t end = rate < 8 * g + 8 ? rate : 8 * g + 8;
+            for (uint32_t i = 8 * g; i < end; i++) columns_add_plane(CP, in[i], i - 8 * g);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int m = 0; m < 4; m++)   // CP[u] byte m = byte g of Q[8m + u]
+                    V[2 * (8 * m + u)] |= ((CP[u] >> (8 * m)) & 0xffu) << (8 * g);
+        }
+#pragma unroll
+        for (int t = 0; t < 32; t++) {
+            const uint32_t v = V[2 * t] * bin;
+            V[2 * t] = v;
+            V[2 * t + 1] = v;
+        }
+        return;
+    }
+    uint32_t D[32];
+    for (uint32_t h = 0; 16 * h < rate; h++) {   // low then high 16-bit halves
+        uint32_t CL[16], CH[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) { CL[j] = 0; CH[j] = 0; }
+        const uint32_t e1 = rate < 16 * h + 8 ? rate : 16 * h + 8;
+        const uint32_t e2 = rate < 16 * h + 16 ? rate : 16 * h + 16;
+        for (uint32_t i = 16 * h; i < e1; i++) {
+            columns_add_plane(CL, in[2 * i], i - 16 * h);
+            columns_add_plane(CL + 8, in[2 * i + 1], i - 16 * h);
+        }
+        for (uint32_t i = 16 * h + 8; i < e2; i++) {
+            columns_add_plane(CH, in[2 * i], i - 16 * h - 8);
+            columns_add_plane(CH + 8, in[2 * i + 1], i - 16 * h - 8);
+        }
+        rows_from_columns(CL, CH, D);
+#pragma unroll
+        for (int i = 0; i < 32; i++) {   // pair i: elements 2i (low half of D) and 2i+1
+            V[2 * i] |= (D[i] & 0xffffu) << (16 * h);
+            V[2 * i + 1] |= (D[i] >> 16) << (16 * h);
+        }
+    }
+    if (bin != 1u) {
+#pragma unroll
+        for (int e = 0; e < 64; e++) V[e] *= bin;
     }
 }
 

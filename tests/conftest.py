@@ -68,6 +68,10 @@ def block_host():
     lib.bh_encode.argtypes = [u16p, u32p, ctypes.c_float, u8p, u32p]
     lib.bh_decode.restype = None
     lib.bh_decode.argtypes = [ctypes.c_uint32, u8p, u32p, u16p]
+    lib.bh_encode_u32.restype = ctypes.c_uint32
+    lib.bh_encode_u32.argtypes = [u32p, u32p, ctypes.c_float, u8p, u32p, u32p]
+    lib.bh_decode_u32.restype = None
+    lib.bh_decode_u32.argtypes = [ctypes.c_uint32, u8p, u32p, u32p]
     lib.bh_fdiv.restype = ctypes.c_uint32
     lib.bh_fdiv.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     lib.bh_quant_mismatches.restype = ctypes.c_uint64

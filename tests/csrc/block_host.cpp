@@ -51,6 +51,35 @@ void bh_decode(uint32_t hdr, const uint8_t *payload, const uint32_t bins[4], uin
         }
 }
 
+// uint32 block: header byte (*rate32 set when the rate would be 32) and payload.
+uint32_t bh_encode_u32(const uint32_t v[64], const uint32_t bins[4], float p, uint8_t *out, uint32_t *hdr,
+                       uint32_t *rate32)
+{
+    Decide d = make_decide(bins, p);
+    uint32_t V[64];
+    for (int i = 0; i < 64; i++) V[i] = v[i];
+    bool r32 = false;
+    uint32_t h = block_header_u32(V, d, &r32);
+    *rate32 = r32 ? 1u : 0u;
+    *hdr = h;
+    if (r32) return 0;
+    uint32_t words[62];
+    uint32_t n = encode_block_u32(V, h, d, words);
+    for (uint32_t i = 0; i < n; i++)
+        for (int b = 0; b < 4; b++) out[4 * i + b] = (uint8_t)(words[i] >> (8 * b));
+    return 4 * n;
+}
+
+void bh_decode_u32(uint32_t hdr, const uint8_t *payload, const uint32_t bins[4], uint32_t v[64])
+{
+    uint32_t words[62] = {0};
+    uint32_t n = header_payload_bytes(hdr) / 4;
+    for (uint32_t i = 0; i < n && i < 62; i++)
+        words[i] = (uint32_t)payload[4 * i] | ((uint32_t)payload[4 * i + 1] << 8) |
+                   ((uint32_t)payload[4 * i + 2] << 16) | ((uint32_t)payload[4 * i + 3] << 24);
+    decode_block_u32(words, hdr, bins, v);
+}
+
 uint32_t bh_fdiv(uint32_t n, uint32_t d) { return fdiv(n, make_fastdiv(d)); }
 
 // Counts (v, d) pairs with v in [0, 65536) and d in [d_lo, d_hi] where the

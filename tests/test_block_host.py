@@ -98,3 +98,52 @@ def test_quantize_exhaustive(block_host):
     assert block_host.bh_quant_mismatches(1, 65536) == 0
     assert block_host.bh_quant_mismatches(65537, 66000) == 0
     assert block_host.bh_quant_mismatches(4294967000, 4294967295) == 0
+
+
+def _blocks_u32(rng, n):
+    out = []
+    for i in range(n):
+        kind = i % 6
+        if kind == 0:
+            v = rng.poisson(0.3, 64) * rng.integers(1, 1 << 24)
+        elif kind == 1:
+            v = rng.integers(0, 1 << 31, 64)
+        elif kind == 2:   # pairs whose sum wraps modulo 2^32
+            v = rng.integers((1 << 32) - 1000, 1 << 32, 64) * (rng.random(64) < 0.5)
+        elif kind == 3:
+            v = rng.integers(0, 2 ** int(rng.integers(1, 33)), 64)
+        elif kind == 4:
+            v = np.zeros(64, np.int64)
+            v[rng.integers(0, 64)] = rng.integers(0, 1 << 32)
+        else:
+            v = rng.integers(0, 1 << 16, 64)
+        out.append(np.asarray(v, np.uint64).astype(np.uint32))
+    return out
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_encode_decode_u32_vs_oracle(block_host, orc, mode):
+    """uint32 blocks: header, payload and decoded values equal the oracle's (rate-32 blocks are flagged)."""
+    bins, p = MODES[mode]
+    rng = np.random.default_rng(7 + sum(map(ord, mode)))
+    for bins_ in (bins, (1000, 5000, 70000, 1 << 20)):
+        b = np.array(bins_, np.uint32)
+        for v in _blocks_u32(rng, 1500):
+            h, pb = orc.block_header(v, b, p)
+            rate = pb // (4 if h & 0x80 else 8)
+            out = np.zeros(256, np.uint8)
+            hh, r32 = ctypes.c_uint32(0), ctypes.c_uint32(0)
+            n = block_host.bh_encode_u32(v.ctypes.data_as(u32p), b.ctypes.data_as(u32p), ctypes.c_float(p),
+                                         out.ctypes.data_as(u8p), ctypes.byref(hh), ctypes.byref(r32))
+            assert bool(r32.value) == (rate == 32)
+            if rate == 32:
+                continue
+            assert hh.value == h, (v, hex(h), hex(hh.value))
+            ref = orc.encode_block(v, h, b)
+            assert out[:n].tobytes() == ref
+            pl = np.zeros(256, np.uint8)
+            pl[: len(ref)] = np.frombuffer(ref, np.uint8)
+            dec = np.zeros(64, np.uint32)
+            block_host.bh_decode_u32(h, pl.ctypes.data_as(u8p), b.ctypes.data_as(u32p), dec.ctypes.data_as(u32p))
+            assert np.array_equal(dec, orc.decode_block(h, ref, b))
+
