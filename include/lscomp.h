@@ -47,7 +47,8 @@ enum {
     LSCOMP_ERR_CORRUPT = 3,      /* stream shorter than its headers say, or a header rate > 16 */
     LSCOMP_ERR_HIP = 4,          /* a HIP runtime call failed */
     LSCOMP_ERR_TOO_LARGE = 5,    /* more than 2^31-1 blocks in one volume */
-    LSCOMP_ERR_INTERNAL = 6      /* look-back did not resolve (should not happen) */
+    LSCOMP_ERR_INTERNAL = 6,     /* look-back did not resolve (should not happen) */
+    LSCOMP_ERR_RANGE = 7         /* uint32: a block's quantized maximum needs 32 bits (rate 32) */
 };
 
 /* Number of 8x8 blocks, x*ceil(y/8)*ceil(z/8) (cuLSZ_entry.cu:97-99). */
@@ -237,6 +238,28 @@ typedef struct lscomp_error_stats {
 
 int lscomp_error_stats_u16(const uint16_t *d_orig, const uint16_t *d_dec, size_t n, lscomp_error_stats *out,
                            hipStream_t stream);
+
+/* ---- uint32 volumes --------------------------------------------------------
+ * Replace cuLSZ_compression_uint32_bsize64 / cuLSZ_decompression_uint32_bsize64
+ * (lsCOMP/include/cuLSZ_entry.h:8-9, lsCOMP/src/cuLSZ_entry.cu:18-91): same
+ * blocking, decision and stream format as uint16 with 32-bit values (the pair
+ * average wraps modulo 2^32 like the reference's uint sum, rates up to 31).
+ * A block whose quantized maximum needs 32 bits cannot be represented (the
+ * reference corrupts its header and leaves an unwritten gap,
+ * cuLSZ_kernel.cu:120-131, 259-261): compression returns LSCOMP_ERR_RANGE
+ * (use larger bins).  Same semantics as the uint16 entries otherwise; the
+ * volume and stream pointers must be 4-byte aligned.  Worst case stream:
+ * lscomp_max_compressed_bytes_u32 = R + 248 per block.  The header and block
+ * counts are those of lscomp_header_bytes_u16 / lscomp_num_blocks_u16. */
+size_t lscomp_max_compressed_bytes_u32(lscomp_dim3 dims);
+int lscomp_compress_u32(const uint32_t *d_in, uint8_t *d_out, size_t out_capacity, size_t *out_bytes,
+                        lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream);
+int lscomp_compress_u32_async(const uint32_t *d_in, uint8_t *d_out, size_t out_capacity, uint64_t *d_out_bytes,
+                              lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream);
+int lscomp_decompress_u32(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                          lscomp_bins4 bins, hipStream_t stream);
+int lscomp_decompress_u32_async(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                                lscomp_bins4 bins, hipStream_t stream);
 
 /* Measurement helper: a plain device copy of `bytes` bytes (d_dst, d_src
  * 16-byte aligned, non-overlapping), one 16-byte load and store per lane like

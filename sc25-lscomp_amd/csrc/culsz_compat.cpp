@@ -1,5 +1,5 @@
 // culsz_compat.cpp -- reference-named C++ entry points over the C ABI.
-// Mirrors lsCOMP/src/cuLSZ_entry.cu:93-166 (names, parameter order, the
+// Mirrors lsCOMP/src/cuLSZ_entry.cu:18-166 (names, parameter order, the
 // print-and-exit error convention of cuLSZ_entry.cu:9-14).
 #include <cstdio>
 #include <cstdlib>
@@ -34,4 +34,26 @@ void cuLSZ_decompression_uint16_bsize64(uint16_t *d_decData, unsigned char *d_cm
     const lscomp_bins4 b = {quantBins.x, quantBins.y, quantBins.z, quantBins.w};
     int st = lscomp_decompress_u16(d_decData, d_cmpBytes, cmpSize, d, b, stream);
     culsz_check(st, "cuLSZ_decompression_uint16_bsize64");
+}
+
+void cuLSZ_compression_uint32_bsize64(uint32_t *d_oriData, unsigned char *d_cmpBytes, size_t *cmpSize,
+                                      uint3 dims, uint4 quantBins, float poolingTH, hipStream_t stream)
+{
+    const lscomp_dim3 d = {dims.x, dims.y, dims.z};
+    const lscomp_bins4 b = {quantBins.x, quantBins.y, quantBins.z, quantBins.w};
+    size_t n = 0;
+    int st = lscomp_compress_u32(d_oriData, d_cmpBytes, lscomp_max_compressed_bytes_u32(d), &n, d, b,
+                                 poolingTH, stream);
+    culsz_check(st, "cuLSZ_compression_uint32_bsize64");
+    *cmpSize = n;
+}
+
+void cuLSZ_decompression_uint32_bsize64(uint32_t *d_decData, unsigned char *d_cmpBytes, size_t cmpSize,
+                                        uint3 dims, uint4 quantBins, float poolingTH, hipStream_t stream)
+{
+    (void)poolingTH;   // unused on decode, as in the reference
+    const lscomp_dim3 d = {dims.x, dims.y, dims.z};
+    const lscomp_bins4 b = {quantBins.x, quantBins.y, quantBins.z, quantBins.w};
+    int st = lscomp_decompress_u32(d_decData, d_cmpBytes, cmpSize, d, b, stream);
+    culsz_check(st, "cuLSZ_decompression_uint32_bsize64");
 }

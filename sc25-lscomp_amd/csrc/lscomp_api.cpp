@@ -73,6 +73,7 @@ int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it, uint32_t t
     const uint64_t nb = num_blocks(dims);
     if (nb > kMaxBlocksPerItem) return LSCOMP_ERR_TOO_LARGE;
     std::memset(&it, 0, sizeof it);
+    it.elem = 2;
     it.x = dims.x;
     it.y = dims.y;
     it.z = dims.z;
@@ -221,6 +222,7 @@ int take_errors(Scratch *s)
     st = hip_fail(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     if (st) return st;
     if (e & ERRF_TIMEOUT) return LSCOMP_ERR_INTERNAL;
+    if (e & ERRF_RANGE) return LSCOMP_ERR_RANGE;
     return LSCOMP_ERR_CORRUPT;
 }
 
@@ -251,6 +253,65 @@ int prepare_decompress(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, ls
     it.s_len = in_bytes;
     it.align = align_class(d_out, dims.z);
     return LSCOMP_OK;
+}
+
+// uint32 volumes: validation and item (tiles of kTileU32 blocks; no HIP calls).
+int prepare_compress_u32(const uint32_t *d_in, uint8_t *d_out, size_t cap, lscomp_dim3 dims, lscomp_bins4 bins,
+                         float p, Item &it)
+{
+    int st = make_item(dims, bins, p, it, kTileU32);
+    if (st) return st;
+    if (!d_in || !d_out || ((uintptr_t)d_in & 3) || ((uintptr_t)d_out & 3)) return LSCOMP_ERR_INVALID_ARG;
+    if (cap < it.hdr_bytes) return LSCOMP_ERR_CAPACITY;
+    it.vol_in = (const uint16_t *)d_in;
+    it.s_out = d_out;
+    it.s_len = cap;
+    it.elem = 4;
+    return LSCOMP_OK;
+}
+
+int prepare_decompress_u32(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                           lscomp_bins4 bins, Item &it)
+{
+    int st = make_item(dims, bins, 0.0f, it, kTileU32);
+    if (st) return st;
+    if (!d_in || !d_out || ((uintptr_t)d_out & 3) || ((uintptr_t)d_in & 3)) return LSCOMP_ERR_INVALID_ARG;
+    if (in_bytes < it.hdr_bytes) return LSCOMP_ERR_CORRUPT;
+    it.vol_out = (uint16_t *)d_out;
+    it.s_in = d_in;
+    it.s_len = in_bytes;
+    it.elem = 4;
+    return LSCOMP_OK;
+}
+
+// Caller holds s->mu.  A uint32 item; decompression always uses the offsets pre-pass.
+int launch_item_u32(Scratch *s, Item &it, bool compress)
+{
+    int st = ensure_tiles(s, it.ntiles);
+    if (st) return st;
+    st = count_launch(s);
+    if (st) return st;
+    static std::atomic<int> occ[2];
+    int o = occ[compress].load();
+    if (o == 0) {
+        o = workgroups_per_cu_u32(compress);
+        occ[compress].store(o);
+    }
+    LaunchCfg cfg{};
+    cfg.tile = kTileU32;
+    cfg.total_tiles = it.ntiles;
+    uint64_t g = (uint64_t)(num_cus() > 0 ? num_cus() : 1) * (uint64_t)o;
+    if (g > it.ntiles) g = it.ntiles;
+    cfg.grid = (uint32_t)(g ? g : 1);
+    if (compress) {
+        if (!it.result) it.result = &s->ctrl->result;
+    } else {
+        cfg.tile_off = s->tile_off;
+        st = count_launch(s);   // the pre-pass is a launch of its own
+        if (st) return st;
+    }
+    return hip_fail(launch_u32(compress, it, cfg, s->ctrl, s->status, s->stream),
+                    compress ? "compress launch" : "decompress launch");
 }
 
 // Caller holds s->mu.  it.ntiles was computed for the direction's tile size.
@@ -352,6 +413,75 @@ int lscomp_decompress_u16(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes,
     return take_errors(s);
 }
 
+size_t lscomp_max_compressed_bytes_u32(lscomp_dim3 dims)
+{
+    const uint64_t nb = num_blocks(dims);
+    return (size_t)((nb + 3) / 4 * 4 + 248 * nb);
+}
+
+int lscomp_compress_u32_async(const uint32_t *d_in, uint8_t *d_out, size_t out_capacity, uint64_t *d_out_bytes,
+                              lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream)
+{
+    if (!d_out_bytes || ((uintptr_t)d_out_bytes & 7)) return LSCOMP_ERR_INVALID_ARG;
+    Item it;
+    int st = prepare_compress_u32(d_in, d_out, out_capacity, dims, bins, pooling_th, it);
+    if (st) return st;
+    it.result = d_out_bytes;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return launch_item_u32(s, it, true);
+}
+
+int lscomp_compress_u32(const uint32_t *d_in, uint8_t *d_out, size_t out_capacity, size_t *out_bytes,
+                        lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream)
+{
+    if (!out_bytes) return LSCOMP_ERR_INVALID_ARG;
+    Item it;
+    int st = prepare_compress_u32(d_in, d_out, out_capacity, dims, bins, pooling_th, it);
+    if (st == LSCOMP_ERR_CAPACITY) *out_bytes = lscomp_header_bytes_u16(dims);
+    if (st) return st;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    st = launch_item_u32(s, it, true);
+    if (st) return st;
+    st = read_ctrl(s);
+    if (st) return st;
+    *out_bytes = (size_t)s->host->result;
+    st = take_errors(s);
+    if (st) return st;
+    return *out_bytes > out_capacity ? LSCOMP_ERR_CAPACITY : LSCOMP_OK;
+}
+
+int lscomp_decompress_u32_async(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                                lscomp_bins4 bins, hipStream_t stream)
+{
+    Item it;
+    int st = prepare_decompress_u32(d_out, d_in, in_bytes, dims, bins, it);
+    if (st) return st;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return launch_item_u32(s, it, false);
+}
+
+int lscomp_decompress_u32(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                          lscomp_bins4 bins, hipStream_t stream)
+{
+    Item it;
+    int st = prepare_decompress_u32(d_out, d_in, in_bytes, dims, bins, it);
+    if (st) return st;
+    Scratch *s = get_scratch(stream);
+    if (!s) return LSCOMP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    st = launch_item_u32(s, it, false);
+    if (st) return st;
+    st = read_ctrl(s);
+    if (st) return st;
+    return take_errors(s);
+}
+
 int lscomp_stream_status(hipStream_t stream)
 {
     Scratch *s = get_scratch(stream);
@@ -425,6 +555,7 @@ const char *lscomp_status_string(int status)
     case LSCOMP_ERR_HIP: return "HIP runtime error";
     case LSCOMP_ERR_TOO_LARGE: return "volume has more than 2^31-1 blocks";
     case LSCOMP_ERR_INTERNAL: return "internal error (look-back timeout)";
+    case LSCOMP_ERR_RANGE: return "uint32 value range needs rate 32, which the format cannot hold";
     default: return "unknown status";
     }
 }

@@ -34,7 +34,8 @@ struct Ctrl {
     uint64_t pad3[14];
 };
 static_assert(sizeof(Ctrl) == 384, "Ctrl is three 128-byte lines");
-enum : uint32_t { ERRF_CORRUPT = 1u, ERRF_TIMEOUT = 2u };
+// ERRF_RANGE: a uint32 block's quantized maximum needs 32 bits (rate 32).
+enum : uint32_t { ERRF_CORRUPT = 1u, ERRF_TIMEOUT = 2u, ERRF_RANGE = 4u };
 
 // Look-back word: epoch (26 bits) | flag (2 bits) | value in 4-byte words (36 bits).
 constexpr uint32_t kEpochMask = (1u << 26) - 1u;
@@ -56,7 +57,8 @@ struct Item {
     uint32_t first_tile;      // first global tile of this item
     uint32_t ntiles;          // ceil(nblocks / tile)
     uint32_t align;           // ALIGN*
-    uint32_t pad0, pad1;
+    uint32_t elem;            // element bytes: 2 (uint16) or 4 (uint32; vol_in/vol_out hold uint32)
+    uint32_t pad1;
     FastDiv div_frame, div_nbz;
     Decide dec;
 };
@@ -78,7 +80,13 @@ hipError_t launch_compress(const Item &single, const Item *items, const uint32_t
 hipError_t launch_decompress(const Item &single, const Item *items, const uint32_t *tile_items,
                              LaunchCfg cfg, Ctrl *ctrl, uint64_t *status, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t value, hipStream_t stream);
+// uint32 volumes: one item, tiles of kTileU32 blocks; decompression always
+// computes tile offsets first (cfg.tile_off must be set).
+constexpr uint32_t kTileU32 = 256;
+hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,
+                      hipStream_t stream);
 int workgroups_per_cu(bool compress, bool table, int tile);   // occupancy limit
+int workgroups_per_cu_u32(bool compress);
 int num_cus();
 
 }  // namespace lsc

@@ -811,6 +811,206 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// uint32 volumes (replace cuLSZ_{compression,decompression}_kernel_uint32_bsize64,
+// lsCOMP/src/cuLSZ_kernel.cu:4-824).  Same tile scheme as uint16 with one block
+// of 64 uint32 values per lane: dynamic tile ids, header bytes, workgroup
+// scan, aggregate published before the look-back, payload packed in LDS and
+// stored with 16-byte stores; decompression uses the tile offsets of
+// offsets_kernel (the header format is the same).  A block whose quantized
+// maximum needs 32 bits flags the launch (ERRF_RANGE).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kWordsU32 = 62;   // payload words of a block at most (31 planes x 8 bytes)
+
+template <int TILE>
+struct TileShared32 {
+    alignas(16) uint32_t pay[TILE * kWordsU32 + 8];
+    uint32_t hdr[TILE / 4];
+    uint32_t wsum[TILE / 64];
+    LbShared<TILE / 64 * lb_per(TILE)> lb;
+    uint32_t tile, epoch;
+};
+
+__device__ __forceinline__ void load_block_u32(const Item &it, uint32_t bx, uint32_t by, uint32_t bz,
+                                               uint32_t V[64])
+{
+    const uint32_t y0 = by * 8, z0 = bz * 8;
+    const uint32_t *p = (const uint32_t *)it.vol_in + ((uint64_t)bx * it.y + y0) * it.z + z0;
+    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
+    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
+    if (ny == 8u && nz == 8u) {
+        // two 16-byte loads per row (dword alignment suffices)
+        uint4 t[16];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            t[2 * r] = *(const uint4 *)(p + (uint64_t)r * it.z);
+            t[2 * r + 1] = *(const uint4 *)(p + (uint64_t)r * it.z + 4);
+        }
+#pragma unroll
+        for (int h = 0; h < 16; h++) {
+            V[4 * h] = t[h].x; V[4 * h + 1] = t[h].y; V[4 * h + 2] = t[h].z; V[4 * h + 3] = t[h].w;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                V[8 * r + k] = ((uint32_t)r < ny && (uint32_t)k < nz) ? p[(uint64_t)r * it.z + k] : 0u;
+    }
+}
+
+__device__ __forceinline__ void store_block_u32(const Item &it, uint32_t bx, uint32_t by, uint32_t bz,
+                                                const uint32_t V[64])
+{
+    const uint32_t y0 = by * 8, z0 = bz * 8;
+    uint32_t *p = (uint32_t *)it.vol_out + ((uint64_t)bx * it.y + y0) * it.z + z0;
+    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
+    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
+    if (ny == 8u && nz == 8u) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            *(uint4 *)(p + (uint64_t)r * it.z) = make_uint4(V[8 * r], V[8 * r + 1], V[8 * r + 2], V[8 * r + 3]);
+            *(uint4 *)(p + (uint64_t)r * it.z + 4) =
+                make_uint4(V[8 * r + 4], V[8 * r + 5], V[8 * r + 6], V[8 * r + 7]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if ((uint32_t)r < ny && (uint32_t)k < nz) p[(uint64_t)r * it.z + k] = V[8 * r + k];
+    }
+}
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_u32_kernel(
+    const Item it, Ctrl *ctrl, uint64_t *status)
+{
+    __shared__ TileShared32<TILE> sh;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t epoch = sh.epoch, ntot = it.ntiles;
+    for (;;) {
+        const uint32_t tile = sh.tile;
+        if (tile >= ntot) break;
+        const uint32_t b = tile * (uint32_t)TILE + tid;
+        const bool valid = b < it.nblocks;
+        uint32_t V[64];
+        uint32_t hdr = 0;
+        if (valid) {
+            uint32_t bx, by, bz;
+            block_coords(it, b, bx, by, bz);
+            load_block_u32(it, bx, by, bz, V);
+            bool rate32 = false;
+            hdr = block_header_u32(V, it.dec, &rate32);
+            if (rate32) {   // not representable: flag the launch, keep the stream consistent
+                __hip_atomic_fetch_or(&ctrl->error, ERRF_RANGE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                hdr &= 0xe0u;
+            }
+        }
+        const uint32_t words = header_payload_bytes(hdr) >> 2;
+        ((uint8_t *)sh.hdr)[tid] = (uint8_t)hdr;
+        uint32_t agg;
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
+        if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tile == 0 ? LB_INCL : LB_AGG, agg));
+        if (words) encode_block_u32(V, hdr, it.dec, &sh.pay[off]);
+        uint64_t excl = 0;
+        if (tile != 0) {
+            excl = lookback_tile<TILE>(status, tile, 0, epoch, sh.lb, ctrl);
+            if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
+        }
+        __syncthreads();   // payload and header bytes are in LDS
+        const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
+        if (tid < (uint32_t)TILE / 16) {
+            const uint64_t o = (uint64_t)tile * TILE + 16ull * tid;
+            if (o + 16 <= it.hdr_bytes) {
+                *(uint4 *)(it.s_out + o) = make_uint4(sh.hdr[4 * tid], sh.hdr[4 * tid + 1], sh.hdr[4 * tid + 2],
+                                                      sh.hdr[4 * tid + 3]);
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < 4; k++)
+                    if (o + 4 * k < it.hdr_bytes) *(uint32_t *)(it.s_out + o + 4 * k) = sh.hdr[4 * tid + k];
+            }
+        }
+        if (g0 + 4ull * agg <= it.s_len) store_payload<TILE>(sh.pay, agg, it.s_out + g0, tid);
+        if (tid == 0 && tile == ntot - 1 && it.result) *it.result = g0 + 4ull * agg;
+        __syncthreads();   // LDS is free again
+        if (tid == 0)      // no waits remain in this iteration: take the next id
+            sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
+    finish_launch(ctrl, epoch);
+}
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_u32_kernel(
+    const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
+{
+    __shared__ TileShared32<TILE> sh;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t epoch = sh.epoch, ntot = it.ntiles;
+    for (;;) {
+        const uint32_t tile = sh.tile;
+        if (tile >= ntot) break;
+        const uint32_t b = tile * (uint32_t)TILE + tid;
+        const bool valid = b < it.nblocks;
+        const uint32_t hdr = valid ? (uint32_t)it.s_in[b] : 0u;
+        const uint64_t excl = tile_off[tile];
+        uint32_t agg;
+        const uint32_t off = tile_scan<TILE>(header_payload_bytes(hdr) >> 2, tid, sh.wsum, &agg);
+        const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
+        const bool fits = agg <= (uint32_t)TILE * kWordsU32 && g0 + 4ull * agg <= it.s_len;
+        const uint32_t shift = (uint32_t)(g0 & 15) >> 2;
+        if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
+        __syncthreads();
+        if (valid) {
+            uint32_t V[64];
+            if (fits) {
+                decode_block_u32(&sh.pay[shift + off], hdr, it.dec.b, V);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 64; i++) V[i] = 0;
+                __hip_atomic_fetch_or(&ctrl->error, ERRF_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            uint32_t bx, by, bz;
+            block_coords(it, b, bx, by, bz);
+            store_block_u32(it, bx, by, bz, V);
+        }
+        __syncthreads();   // LDS is free again
+        if (tid == 0)
+            sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
+    finish_launch(ctrl, epoch);
+}
+
+hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,
+                      hipStream_t stream)
+{
+    constexpr int T = (int)kTileU32;
+    const dim3 grid(cfg.grid), block(T);
+    if (compress) {
+        hipLaunchKernelGGL((compress_u32_kernel<T>), grid, block, 0, stream, it, ctrl, status);
+    } else {
+        if (!cfg.tile_off) return hipErrorInvalidValue;
+        const uint32_t nchunks = (it.ntiles + 63) / 64;
+        const uint32_t g = nchunks < cfg.grid ? nchunks : cfg.grid;
+        hipLaunchKernelGGL((offsets_kernel<T>), dim3(g), dim3(256), 0, stream, it, ctrl, status, cfg.tile_off);
+        hipLaunchKernelGGL((decompress_u32_kernel<T>), grid, block, 0, stream, it, ctrl,
+                           (const uint64_t *)cfg.tile_off);
+    }
+    return hipGetLastError();
+}
+
 static hipError_t launch(bool compress, const Item &single, const Item *items, const uint32_t *tile_items, LaunchCfg cfg,
                          Ctrl *ctrl, uint64_t *status, hipStream_t stream)
 {
@@ -865,6 +1065,16 @@ int workgroups_per_cu(bool compress, bool table, int tile)
     case 512: return per_cu_t<512>(compress, table);
     default: return 1;
     }
+}
+
+int workgroups_per_cu_u32(bool compress)
+{
+    constexpr int T = (int)kTileU32;
+    int per_cu = 0;
+    const hipError_t e = compress
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, compress_u32_kernel<T>, T, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_u32_kernel<T>, T, 0);
+    return (e == hipSuccess && per_cu > 0) ? per_cu : 1;
 }
 
 int num_cus()

@@ -28,6 +28,7 @@ ERR_CORRUPT = 3
 ERR_HIP = 4
 ERR_TOO_LARGE = 5
 ERR_INTERNAL = 6
+ERR_RANGE = 7
 
 LOSSLESS_BINS = (1, 1, 1, 1)
 
@@ -65,6 +66,8 @@ EXPORTS = (
     "lscomp_plan_destroy", "lscomp_server_create", "lscomp_server_submit_u16", "lscomp_server_poll",
     "lscomp_server_release", "lscomp_server_drain", "lscomp_server_replay_u16", "lscomp_server_destroy",
     "lscomp_autotune_u16", "lscomp_set_decode_prepass", "lscomp_error_stats_u16", "lscomp_copy_bytes",
+    "lscomp_max_compressed_bytes_u32", "lscomp_compress_u32", "lscomp_compress_u32_async",
+    "lscomp_decompress_u32", "lscomp_decompress_u32_async",
 )
 
 
@@ -105,6 +108,16 @@ def _load():
     for name in ("lscomp_num_blocks_u16", "lscomp_header_bytes_u16", "lscomp_max_compressed_bytes_u16"):
         getattr(lib, name).restype = sz
         getattr(lib, name).argtypes = [Dim3]
+    lib.lscomp_max_compressed_bytes_u32.restype = sz
+    lib.lscomp_max_compressed_bytes_u32.argtypes = [Dim3]
+    lib.lscomp_compress_u32.restype = ctypes.c_int
+    lib.lscomp_compress_u32.argtypes = [vp, vp, sz, ctypes.POINTER(sz), Dim3, Bins4, ctypes.c_float, vp]
+    lib.lscomp_compress_u32_async.restype = ctypes.c_int
+    lib.lscomp_compress_u32_async.argtypes = [vp, vp, sz, vp, Dim3, Bins4, ctypes.c_float, vp]
+    lib.lscomp_decompress_u32.restype = ctypes.c_int
+    lib.lscomp_decompress_u32.argtypes = [vp, vp, sz, Dim3, Bins4, vp]
+    lib.lscomp_decompress_u32_async.restype = ctypes.c_int
+    lib.lscomp_decompress_u32_async.argtypes = [vp, vp, sz, Dim3, Bins4, vp]
     lib.lscomp_compress_u16.restype = ctypes.c_int
     lib.lscomp_compress_u16.argtypes = [vp, vp, sz, ctypes.POINTER(sz), Dim3, Bins4, ctypes.c_float, vp]
     lib.lscomp_compress_u16_async.restype = ctypes.c_int
@@ -208,12 +221,12 @@ def _stream_handle(stream):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def _check_volume(t, dims):
+def _check_volume(t, dims, esize: int = 2):
     x, y, z = (int(v) for v in dims)
     if not t.is_cuda:
         raise ValueError("volume must be a device tensor")
-    if t.element_size() != 2 or not t.is_contiguous() or t.numel() != x * y * z:
-        raise ValueError("volume must be a contiguous 2-byte tensor of x*y*z elements")
+    if t.element_size() != esize or not t.is_contiguous() or t.numel() != x * y * z:
+        raise ValueError(f"volume must be a contiguous {esize}-byte tensor of x*y*z elements")
 
 
 def compress(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, out=None, stream=None):
@@ -267,6 +280,61 @@ def decompress_async(stream_buf, nbytes: int, dims, bins, out, stream=None) -> N
                                           int(nbytes), _dims(dims), _bins(bins), _stream_handle(stream))
     if st != OK:
         raise LscompError(st, "lscomp_decompress_u16_async")
+
+
+# ---- uint32 volumes (int32-typed tensors carry the uint32 bits) -------------
+
+def max_compressed_bytes_u32(dims) -> int:
+    return int(_lib.lscomp_max_compressed_bytes_u32(_dims(dims)))
+
+
+def compress_u32(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, out=None, stream=None):
+    """Compress a device uint32 volume (4-byte tensor); returns (stream tensor uint8, stream bytes).
+    Raises LscompError (ERR_RANGE when a block would need rate 32)."""
+    import torch
+    _check_volume(vol, dims, 4)
+    if out is None:
+        out = torch.empty(max_compressed_bytes_u32(dims), dtype=torch.uint8, device=vol.device)
+    n = ctypes.c_size_t(0)
+    st = _lib.lscomp_compress_u32(ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                  out.numel(), ctypes.byref(n), _dims(dims), _bins(bins),
+                                  ctypes.c_float(pooling), _stream_handle(stream))
+    if st != OK:
+        err = LscompError(st, "lscomp_compress_u32")
+        err.required = int(n.value)
+        raise err
+    return out, int(n.value)
+
+
+def compress_u32_async(vol, dims, bins, pooling, out, d_size, stream=None) -> None:
+    _check_volume(vol, dims, 4)
+    st = _lib.lscomp_compress_u32_async(ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                        out.numel(), ctypes.c_void_p(d_size.data_ptr()), _dims(dims),
+                                        _bins(bins), ctypes.c_float(pooling), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_compress_u32_async")
+
+
+def decompress_u32(stream_buf, nbytes: int, dims, bins=LOSSLESS_BINS, out=None, stream=None):
+    """Decompress into a (x, y, z) int32-typed volume (uint32 bits)."""
+    import torch
+    x, y, z = (int(v) for v in dims)
+    if out is None:
+        out = torch.empty((x, y, z), dtype=torch.int32, device=stream_buf.device)
+    _check_volume(out, dims, 4)
+    st = _lib.lscomp_decompress_u32(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream_buf.data_ptr()),
+                                    int(nbytes), _dims(dims), _bins(bins), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_decompress_u32")
+    return out
+
+
+def decompress_u32_async(stream_buf, nbytes: int, dims, bins, out, stream=None) -> None:
+    _check_volume(out, dims, 4)
+    st = _lib.lscomp_decompress_u32_async(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream_buf.data_ptr()),
+                                          int(nbytes), _dims(dims), _bins(bins), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_decompress_u32_async")
 
 
 def stream_status(stream=None) -> int:
