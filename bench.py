@@ -152,6 +152,49 @@ def bench_served(torch, lscomp, datagen, np, seconds, rate):
             "budget_bytes": 64 << 30, "slots": 4, "verified": bool(ok)}
 
 
+def bench_u32(torch, lscomp, d_in16, dims, d_out16, args):
+    """The uint32 path (the reference's lsCOMP_cssi path) on the same volume stored
+    as uint32 (4 bytes per pixel).  Its stream must equal the uint16 stream of
+    the same values."""
+    res = {}
+    vol32 = (d_in16.to(torch.int32) & 0xFFFF).contiguous()
+    in_bytes = 4 * vol32.numel()
+    d_out = torch.empty(lscomp.max_compressed_bytes_u32(dims), dtype=torch.uint8, device="cuda")
+    d_dec = torch.empty_like(vol32)
+    d_size = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for name, bins, pool in (("u32_lossless", (1, 1, 1, 1), 1.0), ("u32_lossy_b3-5-10-15_p0.5", (3, 5, 10, 15), 0.5)):
+        comp = lambda: lscomp.compress_u32_async(vol32, dims, bins, pool, d_out, d_size)
+        for _ in range(args.warmup):
+            comp()
+        torch.cuda.synchronize()
+        c_wall, c_ev, _ = timed_loop(torch, comp, args.steps)
+        s_bytes = int(d_size.item())
+        dec = lambda: lscomp.decompress_u32_async(d_out, s_bytes, dims, bins, d_dec)
+        for _ in range(args.warmup):
+            dec()
+        torch.cuda.synchronize()
+        d_wall, d_ev, _ = timed_loop(torch, dec, args.steps)
+        ok = lscomp.stream_status() == lscomp.OK
+        # same values as the uint16 volume: same stream
+        _, n16 = lscomp.compress(d_in16, dims, bins, pool, out=d_out16)
+        ok = ok and n16 == s_bytes and bool(torch.equal(d_out16[:n16], d_out[:s_bytes]))
+        if bins == (1, 1, 1, 1):
+            ok = ok and bool(torch.equal(d_dec, vol32))
+        res[name] = {
+            "compress_GBps": in_bytes / (c_wall * 1e-3) / 1e9,
+            "decompress_GBps": in_bytes / (d_wall * 1e-3) / 1e9,
+            "compress_ms": c_wall, "decompress_ms": d_wall,
+            "compress_kernel_ms": c_ev, "decompress_kernel_ms": d_ev,
+            "stream_bytes": s_bytes, "ratio": in_bytes / s_bytes,
+            "compress_hbm_frac": (in_bytes + s_bytes) / (c_ev * 1e-3) / HBM_PEAK,
+            "decompress_hbm_frac": (in_bytes + s_bytes) / (d_ev * 1e-3) / HBM_PEAK,
+            "verified": ok,
+        }
+        log(f"{name}: {json.dumps(res[name])}")
+    del vol32, d_out, d_dec
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,7 +204,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=48, help="frames of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and served measurements")
-    ap.add_argument("--modes", default="lossless,lossy", help="which settings to time (lossless,lossy)")
+    ap.add_argument("--modes", default="lossless,lossy,u32",
+                    help="which settings to time (lossless,lossy; u32: the same volume as uint32)")
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--served-rate", type=float, default=20000.0)
     args = ap.parse_args()
@@ -225,6 +269,9 @@ def main():
         if name != "lossless":
             results[name]["max_abs_err"] = results_max_err
         log(f"{name}: {json.dumps(results[name])}")
+
+    if "u32" in wanted:
+        results.update(bench_u32(torch, lscomp, d_in, dims, d_out, args))
 
     # measured device-copy bandwidth for context (read + write of the input).
     # The library's plain copy makes 16-byte accesses per lane like the codec
