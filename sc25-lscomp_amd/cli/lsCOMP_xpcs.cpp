@@ -1,188 +1,5 @@
-// lsCOMP_xpcs -- command-line driver of the uint16 codec with the reference
-// CLI's interface (lsCOMP/example/lsCOMP_xpcs.cpp:10-190): the same flags
-// (-i file -d x y z -b x y z w -p th [-o file]), the same validation
-// messages and exit codes, 3 warm-up compressions, then one timed
-// compression and decompression, and the same stdout lines that the AE
-// scripts parse ("cuLSZ compression   end-to-end speed: %f GB/s", three
-// spaces; speeds in 2^20 bytes per ms; ratio = input bytes / stream bytes).
-// Timing brackets one call of the reference-named entry point with hipEvents
-// on the call's stream.  -o writes the decompressed volume.
-#include <hip/hip_runtime.h>
+// lsCOMP_xpcs -- command-line driver of the uint16 codec with the interface of
+// the reference's lsCOMP/example/lsCOMP_xpcs.cpp:10-190 (see lscomp_cli.h).
+#include "lscomp_cli.h"
 
-#include <cstdint>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <vector>
-
-#include "lscomp.h"
-#include "lscomp_culsz.h"
-
-#define HIP_OK(x)                                                                          \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess) {                                                            \
-            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
-            return 1;                                                                      \
-        }                                                                                  \
-    } while (0)
-
-static void usage(const char *prog)
-{
-    std::fprintf(stderr, "Usage:\n");
-    std::fprintf(stderr, "   %s -i oriFilePath -d dims.x dims.y dims.z -b quantBins.x quantBins.y quantBins.z quantBins.w -p value -o decFilePath\n", prog);
-    std::fprintf(stderr, "Options:\n");
-    std::fprintf(stderr, "   -i oriFilePath: Path to the original data file\n");
-    std::fprintf(stderr, "   -d dims.x dims.y dims.z: Dimensions of the original data, where dim.z is the fastest dimension.\n");
-    std::fprintf(stderr, "   -b quantBins.x quantBins.y quantBins.z quantBins.w: Quantization bins for the 4 levels, where x is the base one and x<=y<=z<=w.\n");
-    std::fprintf(stderr, "   -p value: Pooling threshold for a data block.\n");
-    std::fprintf(stderr, "   -o decFilePath: Path to the decompressed data file (optional).\n");
-    std::fprintf(stderr, "Examples:\n");
-    std::fprintf(stderr, "   %s -i data/xpcs.bin -d 1024 1813 1558 -b 3 5 10 15 -p 0.5\n", prog);
-    std::fprintf(stderr, "   %s -i data/xpcs.bin -d 1024 1813 1558 -b 3 5 10 15 -p 0.5 -o data/xpcs-dec.bin\n", prog);
-}
-
-int main(int argc, char *argv[])
-{
-    char ori_path[640] = {0}, dec_path[640] = {0};
-    uint3 dims = {0, 0, 0};
-    uint4 bins = {0, 0, 0, 0};
-    float pooling = 0.0f;
-
-    if (argc < 13) {
-        usage(argv[0]);
-        return 1;
-    }
-    for (int i = 1; i < argc; i++) {
-        if (std::strcmp(argv[i], "-i") == 0) {
-            if (i + 1 >= argc) { std::fprintf(stderr, "Error: Missing value for -i\n"); return 1; }
-            std::strncpy(ori_path, argv[++i], sizeof(ori_path) - 1);
-        } else if (std::strcmp(argv[i], "-d") == 0) {
-            if (i + 3 >= argc) { std::fprintf(stderr, "Error: Missing values for -d\n"); return 1; }
-            int a = std::atoi(argv[i + 1]), b = std::atoi(argv[i + 2]), c = std::atoi(argv[i + 3]);
-            if (a <= 0 || b <= 0 || c <= 0) {
-                std::fprintf(stderr, "Error: -d values must be positive integers\n");
-                return 1;
-            }
-            dims.x = (unsigned)a; dims.y = (unsigned)b; dims.z = (unsigned)c;
-            i += 3;
-        } else if (std::strcmp(argv[i], "-b") == 0) {
-            if (i + 4 >= argc) { std::fprintf(stderr, "Error: Missing values for -b\n"); return 1; }
-            int q[4];
-            for (int k = 0; k < 4; k++) q[k] = std::atoi(argv[i + 1 + k]);
-            if (q[0] <= 0 || q[1] <= 0 || q[2] <= 0 || q[3] <= 0) {
-                std::fprintf(stderr, "Error: -b values must be positive integers\n");
-                return 1;
-            }
-            if (!(q[0] <= q[1] && q[1] <= q[2] && q[2] <= q[3])) {
-                std::fprintf(stderr, "Error: -b values must satisfy x <= y <= z <= w\n");
-                return 1;
-            }
-            bins.x = (unsigned)q[0]; bins.y = (unsigned)q[1]; bins.z = (unsigned)q[2]; bins.w = (unsigned)q[3];
-            i += 4;
-        } else if (std::strcmp(argv[i], "-p") == 0) {
-            if (i + 1 >= argc) { std::fprintf(stderr, "Error: Missing value for -p\n"); return 1; }
-            pooling = (float)std::atof(argv[++i]);
-            if (pooling < 0.0f || pooling > 1.0f) {
-                std::fprintf(stderr, "Error: -p value must be between 0 and 1\n");
-                return 1;
-            }
-        } else if (std::strcmp(argv[i], "-o") == 0) {
-            if (i + 1 >= argc) { std::fprintf(stderr, "Error: Missing value for -o\n"); return 1; }
-            std::strncpy(dec_path, argv[++i], sizeof(dec_path) - 1);
-        } else {
-            std::fprintf(stderr, "Error: Unknown argument %s\n", argv[i]);
-            return 1;
-        }
-    }
-
-    // read the raw little-endian uint16 file
-    std::vector<uint16_t> host;
-    size_t n_ele = 0;
-    if (FILE *f = std::fopen(ori_path, "rb")) {
-        std::fseek(f, 0, SEEK_END);
-        long len = std::ftell(f);
-        std::fseek(f, 0, SEEK_SET);
-        n_ele = len > 0 ? (size_t)len / sizeof(uint16_t) : 0;
-        host.resize(n_ele);
-        if (n_ele && std::fread(host.data(), sizeof(uint16_t), n_ele, f) != n_ele) n_ele = 0;
-        std::fclose(f);
-    } else {
-        std::fprintf(stderr, "Error: cannot open %s\n", ori_path);
-    }
-    if (n_ele != (size_t)dims.x * (size_t)dims.y * (size_t)dims.z) {
-        std::fprintf(stderr, "Error: The number of elements in the original data does not match the dimensions\n");
-        return 1;
-    }
-
-    const lscomp_dim3 d3 = {dims.x, dims.y, dims.z};
-    const size_t cap = lscomp_max_compressed_bytes_u16(d3);
-    uint16_t *d_ori = nullptr, *d_dec = nullptr;
-    unsigned char *d_cmp = nullptr;
-    HIP_OK(hipMalloc((void **)&d_ori, n_ele * sizeof(uint16_t)));
-    HIP_OK(hipMemcpy(d_ori, host.data(), n_ele * sizeof(uint16_t), hipMemcpyHostToDevice));
-    HIP_OK(hipMalloc((void **)&d_dec, n_ele * sizeof(uint16_t)));
-    HIP_OK(hipMalloc((void **)&d_cmp, cap));
-
-    hipStream_t stream;
-    HIP_OK(hipStreamCreate(&stream));
-    hipEvent_t t0, t1;
-    HIP_OK(hipEventCreate(&t0));
-    HIP_OK(hipEventCreate(&t1));
-
-    size_t cmp_size = 0;
-    for (int i = 0; i < 3; i++)
-        cuLSZ_compression_uint16_bsize64(d_ori, d_cmp, &cmp_size, dims, bins, pooling, stream);
-    std::printf("GPU warmup finished!\n\n");
-
-    HIP_OK(hipEventRecord(t0, stream));
-    cuLSZ_compression_uint16_bsize64(d_ori, d_cmp, &cmp_size, dims, bins, pooling, stream);
-    HIP_OK(hipEventRecord(t1, stream));
-    HIP_OK(hipEventSynchronize(t1));
-    float cmp_ms = 0.0f;
-    HIP_OK(hipEventElapsedTime(&cmp_ms, t0, t1));
-
-    // stream to the host and back, so the ratio is checked against a real copy
-    std::vector<unsigned char> cmp_host(cmp_size);
-    HIP_OK(hipMemcpy(cmp_host.data(), d_cmp, cmp_size, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemset(d_cmp, 0, cap));
-    HIP_OK(hipMemcpy(d_cmp, cmp_host.data(), cmp_size, hipMemcpyHostToDevice));
-
-    HIP_OK(hipEventRecord(t0, stream));
-    cuLSZ_decompression_uint16_bsize64(d_dec, d_cmp, cmp_size, dims, bins, pooling, stream);
-    HIP_OK(hipEventRecord(t1, stream));
-    HIP_OK(hipEventSynchronize(t1));
-    float dec_ms = 0.0f;
-    HIP_OK(hipEventElapsedTime(&dec_ms, t0, t1));
-
-    const double mib = n_ele * sizeof(uint16_t) / 1024.0 / 1024.0;
-    std::printf("Dataset information:\n");
-    std::printf("  - dims:   %u x %u x %u\n", dims.x, dims.y, dims.z);
-    std::printf("  - length: %zu\n", n_ele);
-    std::printf("  - size:   %f GB\n", n_ele * sizeof(uint16_t) / 1024.0 / 1024.0 / 1024.0);
-    std::printf("Input arguments:\n");
-    std::printf("  - quantBins: %u %u %u %u\n", bins.x, bins.y, bins.z, bins.w);
-    std::printf("  - poolingTH: %f\n\n", pooling);
-    std::printf("cuLSZ compression   end-to-end speed: %f GB/s\n", mib / cmp_ms);
-    std::printf("cuLSZ decompression end-to-end speed: %f GB/s\n", mib / dec_ms);
-    std::printf("cuLSZ compression ratio: %f\n", mib / (cmp_size / 1024.0 / 1024.0));
-
-    int rc = 0;
-    if (std::strlen(dec_path) > 0) {
-        std::vector<uint16_t> dec(n_ele);
-        HIP_OK(hipMemcpy(dec.data(), d_dec, n_ele * sizeof(uint16_t), hipMemcpyDeviceToHost));
-        FILE *f = std::fopen(dec_path, "wb");
-        if (!f || std::fwrite(dec.data(), sizeof(uint16_t), n_ele, f) != n_ele) {
-            std::fprintf(stderr, "Error: cannot write %s\n", dec_path);
-            rc = 1;
-        }
-        if (f) std::fclose(f);
-    }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    (void)hipFree(d_ori);
-    (void)hipFree(d_dec);
-    (void)hipFree(d_cmp);
-    (void)hipStreamDestroy(stream);
-    return rc;
-}
+int main(int argc, char *argv[]) { return cli_main<uint16_t>(argc, argv, "xpcs"); }

@@ -1,6 +1,7 @@
-"""The lsCOMP_xpcs drop-in CLI, driven like the reference's AE scripts
-(1-reproducing-main-3-1.py:8-18): same flags, same stdout lines, and -o writes
-the decompressed volume (exact for lossless, oracle-equal for lossy)."""
+"""The drop-in CLIs lsCOMP_xpcs (uint16) and lsCOMP_cssi (uint32), driven like
+the reference's AE scripts (1-reproducing-main-3-1.py:8-18): same flags, same
+stdout lines, and -o writes the decompressed volume (exact for lossless,
+oracle-equal for lossy)."""
 import os
 import subprocess
 
@@ -12,7 +13,7 @@ from lsc_helpers import synth
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CLI = os.path.join(ROOT, "sc25-lscomp_amd", "cli", "lsCOMP_xpcs")
+CLI_DIR = os.path.join(ROOT, "sc25-lscomp_amd", "cli")
 
 
 def _parse(stdout):
@@ -27,21 +28,29 @@ def _parse(stdout):
     return comp, dec, ratio
 
 
+@pytest.mark.parametrize("name,dtype", [("lsCOMP_xpcs", np.uint16), ("lsCOMP_cssi", np.uint32)])
 @pytest.mark.parametrize("bins,p", [((1, 1, 1, 1), 1.0), ((3, 5, 10, 15), 0.5)])
-def test_cli_roundtrip(orc, tmp_path, bins, p):
-    assert os.path.exists(CLI), "CLI not built"
+def test_cli_roundtrip(orc, tmp_path, bins, p, name, dtype):
+    cli = os.path.join(CLI_DIR, name)
+    assert os.path.exists(cli), "CLI not built"
     vol = synth("cssi", (4, 181, 155), seed=8)
+    if dtype == np.uint32:   # wider values exercise rates above 16
+        vol = vol.astype(np.uint32) * 4099
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
     vol.tofile(src)
-    cmd = [CLI, "-i", str(src), "-d", *map(str, vol.shape), "-b", *map(str, bins), "-p", str(p), "-o", str(dst)]
+    cmd = [cli, "-i", str(src), "-d", *map(str, vol.shape), "-b", *map(str, bins), "-p", str(p), "-o", str(dst)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     comp, dec, ratio = _parse(r.stdout)
     assert comp > 0 and dec > 0
-    st, ref, refn = orc.compress(vol, bins, p)
-    assert abs(ratio - vol.nbytes / refn) < 1e-4
-    out = np.fromfile(dst, np.uint16).reshape(vol.shape)
-    _, refdec = orc.decompress(ref, vol.shape, bins)
+    if dtype == np.uint16:
+        st, ref, refn = orc.compress(vol, bins, p)
+        _, refdec = orc.decompress(ref, vol.shape, bins)
+    else:
+        st, ref, refn = orc.compress_u32(vol, bins, p)
+        _, refdec = orc.decompress_u32(ref, vol.shape, bins)
+    assert st == 0 and abs(ratio - vol.nbytes / refn) < 1e-4
+    out = np.fromfile(dst, dtype).reshape(vol.shape)
     assert np.array_equal(out, refdec)
     if bins == (1, 1, 1, 1):
         assert np.array_equal(out, vol)
