@@ -261,6 +261,41 @@ int lscomp_decompress_u32(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes,
 int lscomp_decompress_u32_async(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
                                 lscomp_bins4 bins, hipStream_t stream);
 
+/* ---- self-describing container --------------------------------------------
+ * The reference passes dims, bins and stream sizes out of band
+ * (cuLSZ_entry.h:8-11).  A container stores any number of streams (a volume,
+ * or the frames of a batch or of the served form) with their parameters:
+ * a 48-byte header ("LSCP", version 1, item count, total bytes), a 64-byte
+ * little-endian index entry per item, then the streams at 16-byte-aligned
+ * offsets.  Host memory only (copy device streams in and out); no
+ * counterpart in the reference. */
+typedef struct lscomp_container_item {
+    lscomp_dim3 dims;
+    uint32_t elem_bytes;       /* 2: uint16 volume, 4: uint32 volume */
+    lscomp_bins4 bins;
+    float pooling_th;          /* as used for compression (informational) */
+    uint32_t reserved;
+    uint64_t offset;           /* of the stream in the container (set by pack and parse) */
+    uint64_t stream_bytes;
+    uint64_t tag;              /* caller's (e.g. frame number) */
+} lscomp_container_item;
+
+/* Bytes of a container holding these items (their stream_bytes). */
+size_t lscomp_container_bytes(const lscomp_container_item *items, uint32_t n_items);
+/* Writes the container to out (capacity bytes; *out_bytes = its size) from
+ * host streams[i] of items[i].stream_bytes bytes; sets items[i].offset.
+ * LSCOMP_ERR_INVALID_ARG for parameters the codec would reject or a stream
+ * size outside [header bytes, worst case]; LSCOMP_ERR_CAPACITY if too small. */
+int lscomp_container_pack(lscomp_container_item *items, uint32_t n_items, const uint8_t *const *streams,
+                          uint8_t *out, size_t capacity, size_t *out_bytes);
+/* Reads the index of a container of len bytes into items (up to max_items;
+ * *n_items = the container's count).  Stream i is buf + items[i].offset.
+ * LSCOMP_ERR_CORRUPT for a malformed container (magic, version, sizes,
+ * offsets out of order or range, invalid parameters), LSCOMP_ERR_CAPACITY if
+ * max_items is too small. */
+int lscomp_container_parse(const uint8_t *buf, size_t len, lscomp_container_item *items, uint32_t max_items,
+                           uint32_t *n_items);
+
 /* Measurement helper: a plain device copy of `bytes` bytes (d_dst, d_src
  * 16-byte aligned, non-overlapping), one 16-byte load and store per lane like
  * the codec kernels' row accesses.  Gives the copy bandwidth and the scale of

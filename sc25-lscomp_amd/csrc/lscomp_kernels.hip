@@ -821,6 +821,8 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
 // maximum needs 32 bits flags the launch (ERRF_RANGE).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kWordsU32 = 62;   // payload words of a block at most (31 planes x 8 bytes)
+// LDS (62 KB of payload staging per workgroup of 256) allows 2 workgroups per
+// CU, i.e. 2 waves per SIMD, so the kernels may use up to 256 VGPRs.
 
 template <int TILE>
 struct TileShared32 {
@@ -883,7 +885,7 @@ __device__ __forceinline__ void store_block_u32(const Item &it, uint32_t bx, uin
 }
 
 template <int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_u32_kernel(
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void compress_u32_kernel(
     const Item it, Ctrl *ctrl, uint64_t *status)
 {
     __shared__ TileShared32<TILE> sh;
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 }
 
 template <int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_u32_kernel(
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void decompress_u32_kernel(
     const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
 {
     __shared__ TileShared32<TILE> sh;

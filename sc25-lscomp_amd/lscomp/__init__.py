@@ -44,6 +44,15 @@ class Dim3(ctypes.Structure):
     _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32)]
 
 
+class ContainerItem(ctypes.Structure):
+    """lscomp_container_item."""
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32),
+                ("elem_bytes", ctypes.c_uint32), ("b0", ctypes.c_uint32), ("b1", ctypes.c_uint32),
+                ("b2", ctypes.c_uint32), ("b3", ctypes.c_uint32), ("pooling_th", ctypes.c_float),
+                ("reserved", ctypes.c_uint32), ("offset", ctypes.c_uint64), ("stream_bytes", ctypes.c_uint64),
+                ("tag", ctypes.c_uint64)]
+
+
 class Bins4(ctypes.Structure):
     _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32),
                 ("w", ctypes.c_uint32)]
@@ -68,6 +77,7 @@ EXPORTS = (
     "lscomp_autotune_u16", "lscomp_set_decode_prepass", "lscomp_error_stats_u16", "lscomp_copy_bytes",
     "lscomp_max_compressed_bytes_u32", "lscomp_compress_u32", "lscomp_compress_u32_async",
     "lscomp_decompress_u32", "lscomp_decompress_u32_async",
+    "lscomp_container_bytes", "lscomp_container_pack", "lscomp_container_parse",
 )
 
 
@@ -118,6 +128,14 @@ def _load():
     lib.lscomp_decompress_u32.argtypes = [vp, vp, sz, Dim3, Bins4, vp]
     lib.lscomp_decompress_u32_async.restype = ctypes.c_int
     lib.lscomp_decompress_u32_async.argtypes = [vp, vp, sz, Dim3, Bins4, vp]
+    lib.lscomp_container_bytes.restype = sz
+    lib.lscomp_container_bytes.argtypes = [ctypes.POINTER(ContainerItem), ctypes.c_uint32]
+    lib.lscomp_container_pack.restype = ctypes.c_int
+    lib.lscomp_container_pack.argtypes = [ctypes.POINTER(ContainerItem), ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_void_p), vp, sz, ctypes.POINTER(sz)]
+    lib.lscomp_container_parse.restype = ctypes.c_int
+    lib.lscomp_container_parse.argtypes = [vp, sz, ctypes.POINTER(ContainerItem), ctypes.c_uint32,
+                                           ctypes.POINTER(ctypes.c_uint32)]
     lib.lscomp_compress_u16.restype = ctypes.c_int
     lib.lscomp_compress_u16.argtypes = [vp, vp, sz, ctypes.POINTER(sz), Dim3, Bins4, ctypes.c_float, vp]
     lib.lscomp_compress_u16_async.restype = ctypes.c_int
@@ -335,6 +353,81 @@ def decompress_u32_async(stream_buf, nbytes: int, dims, bins, out, stream=None) 
                                           int(nbytes), _dims(dims), _bins(bins), _stream_handle(stream))
     if st != OK:
         raise LscompError(st, "lscomp_decompress_u32_async")
+
+
+# ---- self-describing container (host memory) --------------------------------
+
+def container_pack(entries) -> bytes:
+    """Pack streams with their parameters into one container.
+
+    ``entries``: dicts with ``stream`` (bytes-like, or a uint8 tensor whose first
+    ``stream_bytes`` bytes are the stream), ``stream_bytes`` (optional for
+    bytes-like), ``dims``, ``elem_bytes`` (2 or 4), ``bins``, ``pooling`` and
+    optional ``tag``.  Device tensors are copied to the host."""
+    import numpy as np
+    n = len(entries)
+    items = (ContainerItem * max(n, 1))()
+    bufs = []
+    for i, e in enumerate(entries):
+        s = e["stream"]
+        nbytes = int(e.get("stream_bytes", len(s) if isinstance(s, (bytes, bytearray, memoryview)) else 0))
+        if hasattr(s, "cpu"):
+            s = s[:nbytes].cpu().numpy()
+        arr = np.ascontiguousarray(np.frombuffer(bytes(s[:nbytes]), np.uint8) if not isinstance(s, np.ndarray)
+                                   else s[:nbytes], dtype=np.uint8)
+        bufs.append(arr)
+        x, y, z = (int(v) for v in e["dims"])
+        b = [int(v) for v in e["bins"]]
+        items[i] = ContainerItem(x, y, z, int(e["elem_bytes"]), b[0], b[1], b[2], b[3], float(e["pooling"]), 0, 0,
+                                 arr.size, int(e.get("tag", i)))
+    ptrs = (ctypes.c_void_p * max(n, 1))(*[a.ctypes.data for a in bufs])
+    total = int(_lib.lscomp_container_bytes(items, n))
+    out = np.zeros(max(total, 1), np.uint8)
+    used = ctypes.c_size_t(0)
+    st = _lib.lscomp_container_pack(items, n, ptrs, out.ctypes.data_as(ctypes.c_void_p), out.size,
+                                    ctypes.byref(used))
+    if st != OK:
+        raise LscompError(st, "lscomp_container_pack")
+    return out[: used.value].tobytes()
+
+
+def container_parse(buf):
+    """Index of a container: list of dicts (dims, elem_bytes, bins, pooling, tag,
+    offset, stream_bytes, stream = memoryview into ``buf``)."""
+    import numpy as np
+    arr = np.frombuffer(bytes(buf), np.uint8)
+    n = ctypes.c_uint32(0)
+    st = _lib.lscomp_container_parse(arr.ctypes.data_as(ctypes.c_void_p), arr.size, None, 0, ctypes.byref(n))
+    if st not in (OK, ERR_CAPACITY):
+        raise LscompError(st, "lscomp_container_parse")
+    items = (ContainerItem * max(n.value, 1))()
+    st = _lib.lscomp_container_parse(arr.ctypes.data_as(ctypes.c_void_p), arr.size, items, n.value,
+                                     ctypes.byref(n))
+    if st != OK:
+        raise LscompError(st, "lscomp_container_parse")
+    out = []
+    mv = memoryview(arr)
+    for i in range(n.value):
+        it = items[i]
+        out.append({"dims": (it.x, it.y, it.z), "elem_bytes": it.elem_bytes, "bins": (it.b0, it.b1, it.b2, it.b3),
+                    "pooling": it.pooling_th, "tag": it.tag, "offset": it.offset, "stream_bytes": it.stream_bytes,
+                    "stream": mv[it.offset: it.offset + it.stream_bytes]})
+    return out
+
+
+def container_decompress(buf, device="cuda"):
+    """Decompress every item of a container on the GPU; returns a list of
+    device volumes (int16 tensors for uint16 items, int32 for uint32 items)."""
+    import numpy as np
+    import torch
+    vols = []
+    for e in container_parse(buf):
+        s = torch.from_numpy(np.frombuffer(e["stream"], np.uint8).copy()).to(device)
+        if e["elem_bytes"] == 2:
+            vols.append(decompress(s, e["stream_bytes"], e["dims"], e["bins"]))
+        else:
+            vols.append(decompress_u32(s, e["stream_bytes"], e["dims"], e["bins"]))
+    return vols
 
 
 def stream_status(stream=None) -> int:
