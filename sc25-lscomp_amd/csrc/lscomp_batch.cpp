@@ -113,25 +113,25 @@ int lscomp_plan_create(int direction, uint32_t max_items, uint64_t max_blocks, l
     // every item wastes at most one partial tile
     p->max_tiles = (max_blocks + p->tile - 1) / p->tile + max_items;
     if (p->max_tiles > 0xffffffffull) { delete p; return LSCOMP_ERR_TOO_LARGE; }
-    if (hipGetDevice(&p->device) != hipSuccess) { delete p; return LSCOMP_ERR_HIP; }
+    if (hip_fail(hipGetDevice(&p->device), "hipGetDevice")) { delete p; return LSCOMP_ERR_HIP; }
     p->stage_bytes = table_bytes(max_items, p->max_tiles);
-    bool ok = hipMalloc((void **)&p->d_items, sizeof(Item) * max_items) == hipSuccess &&
-              hipMalloc((void **)&p->d_tile_items, sizeof(uint32_t) * (p->max_tiles + 1)) == hipSuccess &&
-              hipMalloc((void **)&p->d_total, sizeof(uint32_t) * 2) == hipSuccess &&
-              hipMalloc((void **)&p->d_sizes, sizeof(uint64_t) * max_items) == hipSuccess &&
-              hipMalloc((void **)&p->ctrl, sizeof(Ctrl)) == hipSuccess &&
-              hipMalloc((void **)&p->status, sizeof(uint64_t) * p->max_tiles) == hipSuccess &&
-              hipHostMalloc((void **)&p->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) == hipSuccess;
+    bool ok = !hip_fail(hipMalloc((void **)&p->d_items, sizeof(Item) * max_items), "hipMalloc") &&
+              !hip_fail(hipMalloc((void **)&p->d_tile_items, sizeof(uint32_t) * (p->max_tiles + 1)), "hipMalloc") &&
+              !hip_fail(hipMalloc((void **)&p->d_total, sizeof(uint32_t) * 2), "hipMalloc") &&
+              !hip_fail(hipMalloc((void **)&p->d_sizes, sizeof(uint64_t) * max_items), "hipMalloc") &&
+              !hip_fail(hipMalloc((void **)&p->ctrl, sizeof(Ctrl)), "hipMalloc") &&
+              !hip_fail(hipMalloc((void **)&p->status, sizeof(uint64_t) * p->max_tiles), "hipMalloc") &&
+              !hip_fail(hipHostMalloc((void **)&p->h_ctrl, sizeof(Ctrl), hipHostMallocDefault), "hipHostMalloc");
     for (int i = 0; ok && i < 2; i++)
-        ok = hipHostMalloc((void **)&p->h_stage[i], p->stage_bytes, hipHostMallocDefault) == hipSuccess &&
-             hipEventCreateWithFlags(&p->stage_done[i], hipEventDisableTiming) == hipSuccess;
+        ok = !hip_fail(hipHostMalloc((void **)&p->h_stage[i], p->stage_bytes, hipHostMallocDefault), "hipHostMalloc") &&
+             !hip_fail(hipEventCreateWithFlags(&p->stage_done[i], hipEventDisableTiming), "hipEventCreate");
     if (!ok) { free_plan(p); return LSCOMP_ERR_HIP; }
     // zero scratch and tables with a fill kernel, then wait (creation is not stream-ordered)
-    ok = launch_fill_u64((uint64_t *)p->ctrl, sizeof(Ctrl) / 8, 0, nullptr) == hipSuccess &&
-         launch_fill_u64(p->status, p->max_tiles, 0, nullptr) == hipSuccess &&
-         launch_fill_u64((uint64_t *)p->d_total, 1, 0, nullptr) == hipSuccess &&
-         launch_fill_u64(p->d_sizes, max_items, 0, nullptr) == hipSuccess &&
-         hipStreamSynchronize(nullptr) == hipSuccess;
+    ok = !hip_fail(launch_fill_u64((uint64_t *)p->ctrl, sizeof(Ctrl) / 8, 0, nullptr), "fill") &&
+         !hip_fail(launch_fill_u64(p->status, p->max_tiles, 0, nullptr), "fill") &&
+         !hip_fail(launch_fill_u64((uint64_t *)p->d_total, 1, 0, nullptr), "fill") &&
+         !hip_fail(launch_fill_u64(p->d_sizes, max_items, 0, nullptr), "fill") &&
+         !hip_fail(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     if (!ok) { free_plan(p); return LSCOMP_ERR_HIP; }
     *plan = p;
     return LSCOMP_OK;
