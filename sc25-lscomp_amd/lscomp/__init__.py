@@ -111,6 +111,13 @@ class Completion(ctypes.Structure):
 
 
 def _load():
+    # PyTorch ships its own HIP runtime.  If liblscomp.so pulled the system
+    # runtime into the process first, torch's would bind to it and the process
+    # could see no device; importing torch first makes one runtime serve both.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C sc25-lscomp_amd")
     lib = ctypes.CDLL(LIB_PATH)
