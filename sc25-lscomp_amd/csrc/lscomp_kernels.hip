@@ -253,7 +253,7 @@ template <int TILE>
 struct TileShared {
     static constexpr int kWaves = TILE / 64;
     alignas(16) uint32_t pay[TILE * 32 + 8];   // payload staging (128 B per block max) + shift slack
-    uint32_t hdr[2][TILE / 4];     // header bytes, one buffer per pipeline stage
+    uint32_t hdr[3][TILE / 4];     // header bytes, one buffer per pipeline stage
     uint32_t wsum[kWaves];
     LbShared<kWaves * lb_per(TILE)> lb;
     uint32_t tile, next, epoch, ntot;
@@ -430,23 +430,78 @@ __device__ __forceinline__ const Item &item_at(const Item &single, const Item *i
     return items[idx];
 }
 
-// Two-stage pipeline per workgroup.  Iteration k holds tile c (packed in LDS,
-// aggregate published, look-back pending) and takes on tile n:
-//   1. issue the loads of n and the first look-back probe of c, then take the
+// A tile between its packing and its stores: payload staged in LDS at
+// pay[start, start + agg), headers in hdr[hb], aggregate published.
+struct PendTile {
+    uint32_t tile, tl, item, agg, start, hb;
+};
+
+// Start of a run of `len` words in the staging ring [0, ring) behind the
+// pending runs (a = oldest, b = newest when nlive == 2; a run of length 0
+// occupies nothing): after the newest run, or at 0 when that passes the end of
+// the ring and the run fits before the oldest.  ~0u when neither fits.
+__device__ __forceinline__ uint32_t ring_place(uint32_t len, uint32_t nlive, const PendTile &a, const PendTile &b,
+                                               uint32_t ring)
+{
+    const bool ua = nlive >= 1 && a.agg != 0, ub = nlive == 2 && b.agg != 0;
+    if (!ua && !ub) return 0;
+    if (ua != ub) {   // one run
+        const PendTile &r = ua ? a : b;
+        if (r.start + r.agg + len <= ring) return r.start + r.agg;
+        return len <= r.start ? 0u : ~0u;
+    }
+    const uint32_t tail = b.start + b.agg;
+    if (b.start > a.start) {   // in address order
+        if (tail + len <= ring) return tail;
+        return len <= a.start ? 0u : ~0u;
+    }
+    return tail + len <= a.start ? tail : ~0u;   // the newest run wrapped to the front
+}
+
+// Stores a tile whose exclusive prefix (in words) is excl: its TILE header
+// bytes clipped to the padded header region (pad bytes are 0; 16-byte stores,
+// the last partial chunk by dwords) and its payload; the item's last tile also
+// writes the stream size.
+template <int TILE>
+__device__ __forceinline__ void store_tile(const Item &it, uint32_t tl, uint32_t agg, const uint32_t *hdr,
+                                           const uint32_t *pay, uint64_t excl, uint32_t tid)
+{
+    const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;   // byte offset of the tile payload
+    if (tid < (uint32_t)TILE / 16) {
+        const uint64_t o = (uint64_t)tl * TILE + 16ull * tid;
+        if (o + 16 <= it.hdr_bytes) {
+            *(uint4 *)(it.s_out + o) = make_uint4(hdr[4 * tid], hdr[4 * tid + 1], hdr[4 * tid + 2], hdr[4 * tid + 3]);
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++)
+                if (o + 4 * k < it.hdr_bytes) *(uint32_t *)(it.s_out + o + 4 * k) = hdr[4 * tid + k];
+        }
+    }
+    if (g0 + 4ull * agg <= it.s_len) store_payload<TILE>(pay, agg, it.s_out + g0, tid);
+    if (tid == 0 && tl == it.ntiles - 1 && it.result) *it.result = g0 + 4ull * agg;
+}
+
+// Three-stage pipeline per workgroup.  A tile is loaded, scanned and packed
+// into LDS in one iteration (its aggregate published at once) and stays
+// pending for two more; the iteration that takes on tile n finishes the oldest
+// pending tile p (look-back, inclusive prefix, stores):
+//   1. issue the loads of n and the first look-back probe of p, then take the
 //      next tile id (kept in a register of thread 0 until the end of the
 //      iteration, so the atomic's round trip overlaps all of it)
 //   2. header/scan of n, publish n's aggregate
-//   3. pack n into LDS behind c's payload (a ring: the payload staging area
-//      holds both while their sizes fit, the common case), which gives c's
-//      probe time to arrive
-//   4. evaluate c's probe (re-reading while predecessors are still empty),
-//      publish c's inclusive prefix
-//   5. store c's headers and payload; pack n now if it did not fit before.
-// Forward progress: a tile's look-back waits only for lower tile ids, whose
-// owners are running workgroups (ids are taken dynamically), so the lowest
-// unfinished tile can always proceed.  A reserved id (step 1) waits at most
-// for the rest of the iteration before its loads start; timings showed that
-// hiding the atomic is worth more than the occasional longer wait on c.
+//   3. pack n into the LDS ring behind the pending tiles when it fits (the
+//      common case), which gives p's probe time to arrive
+//   4. evaluate p's probe (re-reading while predecessors are still empty),
+//      publish p's inclusive prefix
+//   5. store p's headers and payload; if n did not fit, finish further pending
+//      tiles until it does, then pack it.
+// By the time p is resolved its predecessors have had a whole iteration to
+// publish their aggregates, so the look-back rarely waits, and a workgroup
+// that does wait does not hold up the tiles behind its reserved id for long.
+// Forward progress: a look-back waits only for lower tile ids; the lowest tile
+// without an inclusive prefix belongs to a workgroup that, whatever stage the
+// tile is in, either resolves it (all its predecessors are inclusive) or
+// finishes lower pending tiles of its own first, so it always advances.
 template <int TILE, bool kTable>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(
     const Item single, const Item *__restrict__ items, const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
@@ -463,14 +518,15 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
 
-    bool have_cur = false;
-    uint32_t c_tile = 0, c_tl = 0, c_item = 0, c_agg = 0, c_start = 0, hb = 0;
+    uint32_t np = 0, hb = 0;   // pending tiles (p0 oldest), header buffer of the next tile
+    PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
     for (;;) {
         const uint32_t n = sh.next;
         const bool have_n = n < ntot;
-        if (!have_n && !have_cur) break;
+        if (!have_n && np == 0) break;
+        const bool resolve = np == 2 || (!have_n && np != 0);
 
-        // 1. loads of n, first probe of c, next id
+        // 1. loads of n, first probe of p0, next id
         uint32_t D[32];
         uint32_t n_item = 0, n_tl = 0, nb = 0;
         bool nvalid = false;
@@ -486,13 +542,13 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                 load_block(nit, bx, by, bz, D);
             }
         }
-        const bool c_lb = have_cur && c_tl != 0;   // c needs a look-back
+        const bool p_lb = resolve && p0.tl != 0;   // p0 needs a look-back
         uint64_t w0[lb_per(TILE)];
-        if (c_lb) lb_probe<TILE>(status, (int64_t)c_tile - 1, item_at<kTable>(single, items, c_item).first_tile, epoch, w0);
+        if (p_lb) lb_probe<TILE>(status, (int64_t)p0.tile - 1, item_at<kTable>(single, items, p0.item).first_tile, epoch, w0);
         // Issued last: returns are counted in order, so waiting for the loads or
         // the probe must not include the atomic's longer round trip.
-        uint32_t next_id = 0;
-        if (tid == 0)
+        uint32_t next_id = n;
+        if (tid == 0 && have_n)
             next_id = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
         // 2. header, scan and aggregate of n
@@ -508,70 +564,58 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (tid == 0) st_relaxed(status + n, lb_word(epoch, n_tl == 0 ? LB_INCL : LB_AGG, n_agg));
         }
 
-        // 3. pack n behind c's payload if both fit in the staging ring
-        uint32_t n_start = 0;
-        bool packed = false;
+        // 3. pack n behind the pending tiles if it fits in the staging ring
+        uint32_t n_start = ~0u;
         if (have_n) {
-            if (!have_cur) {
-                packed = true;
-            } else if (c_start + c_agg + n_agg <= kRing) {
-                n_start = c_start + c_agg;
-                packed = true;
-            } else if (n_agg <= c_start) {
-                packed = true;
-            }
-            if (packed && n_words)
+            n_start = ring_place(n_agg, np, p0, p1, kRing);
+            if (n_start != ~0u && n_words)
                 encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
         }
 
-        // 4-5. finish c
-        if (have_cur) {
-            const Item &cit = item_at<kTable>(single, items, c_item);
-            uint64_t c_excl = 0;
-            if (c_lb) {
+        // 4-5. finish p0
+        if (resolve) {
+            const Item &pit = item_at<kTable>(single, items, p0.item);
+            uint64_t excl = 0;
+            if (p_lb) {
                 uint32_t err;
-                const uint32_t r = lb_eval<TILE>(w0, epoch, sh.lb, ctrl, false, 0, c_excl, &err);
+                const uint32_t r = lb_eval<TILE>(w0, epoch, sh.lb, ctrl, false, 0, excl, &err);
                 if (r != LB_DONE) {
-                    const int64_t base = (int64_t)c_tile - 1 - (r == LB_NEXT ? (int64_t)lb_per(TILE) * TILE : 0);
-                    c_excl = lookback_from<TILE>(status, base, cit.first_tile, epoch, sh.lb, ctrl, c_excl,
-                                                 r == LB_RETRY ? 1u : 0u, 1u);
+                    const int64_t base = (int64_t)p0.tile - 1 - (r == LB_NEXT ? (int64_t)lb_per(TILE) * TILE : 0);
+                    excl = lookback_from<TILE>(status, base, pit.first_tile, epoch, sh.lb, ctrl, excl,
+                                               r == LB_RETRY ? 1u : 0u, 1u);
                 }
-                if (tid == 0) st_relaxed(status + c_tile, lb_word(epoch, LB_INCL, c_excl + c_agg));
+                if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
             }
-            const uint32_t *chdr = sh.hdr[hb ^ 1u];
-            const uint64_t g0 = (uint64_t)cit.hdr_bytes + 4ull * c_excl;   // byte offset of the tile payload
-            // header bytes [tl*TILE, tl*TILE + TILE) clipped to R (pad bytes are 0):
-            // 16-byte stores, the last partial chunk by dwords
-            if (tid < (uint32_t)TILE / 16) {
-                const uint64_t o = (uint64_t)c_tl * TILE + 16ull * tid;
-                if (o + 16 <= cit.hdr_bytes) {
-                    *(uint4 *)(cit.s_out + o) = make_uint4(chdr[4 * tid], chdr[4 * tid + 1], chdr[4 * tid + 2],
-                                                           chdr[4 * tid + 3]);
-                } else {
-#pragma unroll
-                    for (uint32_t k = 0; k < 4; k++)
-                        if (o + 4 * k < cit.hdr_bytes) *(uint32_t *)(cit.s_out + o + 4 * k) = chdr[4 * tid + k];
-                }
-            }
-            if (g0 + 4ull * c_agg <= cit.s_len) store_payload<TILE>(sh.pay + c_start, c_agg, cit.s_out + g0, tid);
-            if (tid == 0 && c_tl == cit.ntiles - 1 && cit.result) *cit.result = g0 + 4ull * c_agg;
+            store_tile<TILE>(pit, p0.tl, p0.agg, sh.hdr[p0.hb], sh.pay + p0.start, excl, tid);
+            p0 = p1;
+            np--;
         }
         if (tid == 0) sh.next = next_id;
-        __syncthreads();   // c's payload has left LDS; n's packed payload and sh.next are visible
-        if (have_n && !packed) {   // n did not fit beside c: pack it now, at the start of the ring
-            if (n_words) encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_off]);
+        __syncthreads();   // p0's payload has left LDS; n's packed payload and sh.next are visible
+        if (have_n && n_start == ~0u) {
+            // n did not fit: finish pending tiles (oldest first) until it does
+            for (;;) {
+                n_start = ring_place(n_agg, np, p0, p1, kRing);
+                if (n_start != ~0u) break;
+                const Item &pit = item_at<kTable>(single, items, p0.item);
+                uint64_t excl = 0;
+                if (p0.tl != 0) {
+                    excl = lookback_tile<TILE>(status, p0.tile, pit.first_tile, epoch, sh.lb, ctrl);
+                    if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
+                }
+                store_tile<TILE>(pit, p0.tl, p0.agg, sh.hdr[p0.hb], sh.pay + p0.start, excl, tid);
+                p0 = p1;
+                np--;
+                __syncthreads();
+            }
+            if (n_words) encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
             __syncthreads();
         }
         if (have_n) {
-            have_cur = true;
-            c_tile = n;
-            c_tl = n_tl;
-            c_item = n_item;
-            c_agg = n_agg;
-            c_start = n_start;
-            hb ^= 1u;
-        } else {
-            have_cur = false;
+            const PendTile t = {n, n_tl, n_item, n_agg, n_start, hb};
+            if (np == 0) p0 = t; else p1 = t;
+            np++;
+            hb = hb == 2u ? 0u : hb + 1u;
         }
     }
     finish_launch(ctrl, epoch);
