@@ -41,14 +41,26 @@ __device__ __forceinline__ void st_relaxed(uint64_t *p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Inclusive scan over the wave's 64 lanes with DPP moves: shifts by 1, 2, 4 and
+// 8 lanes within each row of 16, then row 0's (row 2's) last lane added to row
+// 1 (row 3) and lane 31 added to rows 2 and 3.  Lanes a move does not reach
+// add 0.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
 {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += t;
-    }
+    (void)lane;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31
     return v;
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t lane)
+{
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)lane) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)lane);
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
@@ -275,18 +287,17 @@ __device__ __forceinline__ void lb_probe(const uint64_t *status, int64_t base, u
 
 enum : uint32_t { LB_RETRY = 0, LB_NEXT = 1, LB_DONE = 2 };
 
-// Evaluates one probe round (words w[] of the window that ends at the tile
-// being resolved minus base): adds the values up to the nearest inclusive
-// predecessor to excl.  Returns LB_DONE (excl is the exclusive prefix),
-// LB_NEXT (no inclusive predecessor in the window: continue one window further
-// back) or LB_RETRY (an empty predecessor lies before the nearest inclusive
-// one: read the same window again).  *err receives the launch error word when
-// `poll` is set (else 0).  `par` alternates between consecutive calls (LDS
-// buffer).  Called by every thread; one barrier.
+// First half of the evaluation of one probe round (words w[] of the window
+// that ends at the tile being resolved minus base): this wave's slices (64
+// window positions each) go to LDS buffer `par`: nearest inclusive and empty
+// positions and the sum of the values up to the nearest inclusive one (all
+// values without one).  An aggregate is at most 2^20 words (a status word
+// covers at most 64 tiles of at most 512 blocks of at most 32 words, or 256
+// blocks of at most 62), so the aggregate lanes of a slice are summed in 32
+// bits and the inclusive lane's value is added in 64.  No barrier.
 template <int TILE>
-__device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint32_t epoch,
-                                            LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, bool poll,
-                                            uint32_t par, uint64_t &excl, uint32_t *err)
+__device__ __forceinline__ void lb_slices(const uint64_t w[lb_per(TILE)], uint32_t epoch,
+                                          LbShared<TILE / 64 * lb_per(TILE)> &sh, uint32_t par)
 {
     constexpr int P = lb_per(TILE), NW = TILE / 64;
     constexpr uint32_t kWin = (uint32_t)P * TILE;
@@ -295,9 +306,10 @@ __device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint
     for (int k = 0; k < P; k++) {
         const uint32_t flag = ((uint32_t)(w[k] >> 38) == epoch) ? (uint32_t)(w[k] >> 36) & 3u : 0u;
         const uint64_t im = __ballot(flag == LB_INCL), em = __ballot(flag == LB_EMPTY);
-        // slice sum: values of the lanes up to the first inclusive one (all lanes without one)
-        const uint32_t cut = im ? (uint32_t)__builtin_ctzll(im) : 63u;
-        const uint64_t part = wave_sum_u64(lane <= cut ? (w[k] & kValMask) : 0ull);
+        const uint32_t cut = im ? (uint32_t)__builtin_ctzll(im) : 64u;
+        const uint32_t lo = wave_incl_scan(lane < cut ? (uint32_t)w[k] : 0u, lane);
+        uint64_t part = (uint32_t)__builtin_amdgcn_readlane((int)lo, 63);
+        if (im) part += readlane_u64(w[k], cut) & kValMask;
         if (lane == 0) {
             const uint32_t at = (uint32_t)k * TILE + (tid & ~63u);
             sh.pos_incl[par][k * NW + wave] = im ? at + cut : kWin;
@@ -305,26 +317,51 @@ __device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint
             sh.sum[par][k * NW + wave] = part;
         }
     }
-    if (tid == 0)
-        sh.err[par] = poll ? __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-    __syncthreads();
+}
+
+// Second half, after a barrier that follows every wave's lb_slices: adds the
+// values up to the nearest inclusive predecessor to excl.  Returns LB_DONE
+// (excl is the exclusive prefix), LB_NEXT (no inclusive predecessor in the
+// window: continue one window further back) or LB_RETRY (an empty predecessor
+// lies before the nearest inclusive one: read the same window again).
+template <int TILE>
+__device__ __forceinline__ uint32_t lb_reduce(const LbShared<TILE / 64 * lb_per(TILE)> &sh, uint32_t par,
+                                              uint64_t &excl)
+{
+    constexpr int NS = TILE / 64 * lb_per(TILE);
+    constexpr uint32_t kWin = (uint32_t)lb_per(TILE) * TILE;
     uint32_t j = kWin, e = kWin;
-    for (int v = 0; v < P * NW; v++) {
+    for (int v = 0; v < NS; v++) {
         const uint32_t x = sh.pos_incl[par][v], y = sh.pos_empty[par][v];
         j = x < j ? x : j;
         e = y < e ? y : e;
     }
-    *err = sh.err[par];
     // an empty predecessor nearer than the nearest inclusive one (or anywhere,
     // when the window holds no inclusive one) must be read again
     if (e < j) return LB_RETRY;
     // slices are in window order (slice v covers positions 64v .. 64v+63); the
     // slices before j's hold only aggregates, j's slice sums up to j
     uint64_t total = 0;
-    for (int v = 0; v < P * NW; v++)
+    for (int v = 0; v < NS; v++)
         if (64u * (uint32_t)v <= j) total += sh.sum[par][v];
     excl += total;
     return j < kWin ? LB_DONE : LB_NEXT;
+}
+
+// One probe round evaluated by the workgroup (lb_slices, barrier, lb_reduce).
+// *err receives the launch error word when `poll` is set (else 0).  `par`
+// alternates between consecutive calls (LDS buffer).  Called by every thread.
+template <int TILE>
+__device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint32_t epoch,
+                                            LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, bool poll,
+                                            uint32_t par, uint64_t &excl, uint32_t *err)
+{
+    lb_slices<TILE>(w, epoch, sh, par);
+    if (threadIdx.x == 0)
+        sh.err[par] = poll ? __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    __syncthreads();
+    *err = sh.err[par];
+    return lb_reduce<TILE>(sh, par, excl);
 }
 
 // Look-back from the window ending at tile `base` (inclusive) towards the
@@ -484,20 +521,22 @@ __device__ __forceinline__ void store_tile(const Item &it, uint32_t tl, uint32_t
 // Three-stage pipeline per workgroup.  A tile is loaded, scanned and packed
 // into LDS in one iteration (its aggregate published at once) and stays
 // pending for two more; the iteration that takes on tile n finishes the oldest
-// pending tile p (look-back, inclusive prefix, stores):
-//   1. issue the loads of n and the first look-back probe of p, then take the
-//      next tile id (kept in a register of thread 0 until the end of the
-//      iteration, so the atomic's round trip overlaps all of it)
-//   2. header/scan of n, publish n's aggregate
+// pending tile p:
+//   1. issue the loads of n and the look-back probe of p, then take the next
+//      tile id (kept in a register of thread 0 until the next scan, so the
+//      atomic's round trip overlaps the loads)
+//   2. header of n; n's scan and the first half of p's look-back evaluation
+//      share one barrier; publish n's aggregate
 //   3. pack n into the LDS ring behind the pending tiles when it fits (the
-//      common case), which gives p's probe time to arrive
-//   4. evaluate p's probe (re-reading while predecessors are still empty),
-//      publish p's inclusive prefix
-//   5. store p's headers and payload; if n did not fit, finish further pending
-//      tiles until it does, then pack it.
+//      common case)
+//   4. finish p's look-back (re-reading while predecessors are still empty),
+//      publish its inclusive prefix, store its headers and payload; if n did
+//      not fit, finish further pending tiles until it does and pack it.
 // By the time p is resolved its predecessors have had a whole iteration to
 // publish their aggregates, so the look-back rarely waits, and a workgroup
 // that does wait does not hold up the tiles behind its reserved id for long.
+// (Timed alternatives: issuing the next loads before the closing barrier, and
+// storing p before packing n, were both slower.)
 // Forward progress: a look-back waits only for lower tile ids; the lowest tile
 // without an inclusive prefix belongs to a workgroup that, whatever stage the
 // tile is in, either resolves it (all its predecessors are inclusive) or
@@ -509,7 +548,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 {
     constexpr uint32_t kRing = (uint32_t)TILE * 32u;   // staging words: one worst-case tile
     __shared__ TileShared<TILE> sh;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid == 0) {
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
@@ -520,25 +559,25 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 
     uint32_t np = 0, hb = 0;   // pending tiles (p0 oldest), header buffer of the next tile
     PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
+    uint32_t n = sh.next;
     for (;;) {
-        const uint32_t n = sh.next;
         const bool have_n = n < ntot;
         if (!have_n && np == 0) break;
         const bool resolve = np == 2 || (!have_n && np != 0);
 
-        // 1. loads of n, first probe of p0, next id
+        // 1. loads of n, probe of p0, next id
         uint32_t D[32];
-        uint32_t n_item = 0, n_tl = 0, nb = 0;
+        uint32_t n_item = 0, n_tl = 0;
         bool nvalid = false;
         if (have_n) {
             n_item = kTable ? __builtin_amdgcn_readfirstlane(tile_items[n]) : 0u;
             const Item &nit = item_at<kTable>(single, items, n_item);
             n_tl = n - nit.first_tile;
-            nb = n_tl * (uint32_t)TILE + tid;
-            nvalid = nb < nit.nblocks;
+            const uint32_t b = n_tl * (uint32_t)TILE + tid;
+            nvalid = b < nit.nblocks;
             if (nvalid) {
                 uint32_t bx, by, bz;
-                block_coords(nit, nb, bx, by, bz);
+                block_coords(nit, b, bx, by, bz);
                 load_block(nit, bx, by, bz, D);
             }
         }
@@ -551,34 +590,40 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         if (tid == 0 && have_n)
             next_id = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
-        // 2. header, scan and aggregate of n
-        uint32_t n_hdr = 0, n_words = 0, n_off = 0, n_agg = 0;
+        // 2. header of n; n's scan and the first half of p0's evaluation share a barrier
+        uint32_t n_hdr = 0, n_words = 0;
         if (have_n) {
-            const Item &nit = item_at<kTable>(single, items, n_item);
             if (nvalid) {
-                n_hdr = block_header(D, nit.dec);
+                n_hdr = block_header(D, item_at<kTable>(single, items, n_item).dec);
                 n_words = header_payload_bytes(n_hdr) >> 2;
             }
             ((uint8_t *)sh.hdr[hb])[tid] = (uint8_t)n_hdr;
-            n_off = tile_scan<TILE>(n_words, tid, sh.wsum, &n_agg);
-            if (tid == 0) st_relaxed(status + n, lb_word(epoch, n_tl == 0 ? LB_INCL : LB_AGG, n_agg));
         }
+        const uint32_t incl = wave_incl_scan(n_words, lane);
+        if (lane == 63) sh.wsum[wave] = incl;
+        if (p_lb) lb_slices<TILE>(w0, epoch, sh.lb, 0);
+        __syncthreads();
+        uint32_t before = 0, n_agg = 0;
+#pragma unroll
+        for (int v = 0; v < TILE / 64; v++) {
+            const uint32_t x = sh.wsum[v];
+            before += (v < (int)wave) ? x : 0u;
+            n_agg += x;
+        }
+        const uint32_t n_off = before + incl - n_words;
+        if (have_n && tid == 0) st_relaxed(status + n, lb_word(epoch, n_tl == 0 ? LB_INCL : LB_AGG, n_agg));
 
         // 3. pack n behind the pending tiles if it fits in the staging ring
-        uint32_t n_start = ~0u;
-        if (have_n) {
-            n_start = ring_place(n_agg, np, p0, p1, kRing);
-            if (n_start != ~0u && n_words)
-                encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
-        }
+        uint32_t n_start = have_n ? ring_place(n_agg, np, p0, p1, kRing) : ~0u;
+        if (n_start != ~0u && n_words)
+            encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
 
-        // 4-5. finish p0
+        // 4. finish p0
         if (resolve) {
             const Item &pit = item_at<kTable>(single, items, p0.item);
             uint64_t excl = 0;
             if (p_lb) {
-                uint32_t err;
-                const uint32_t r = lb_eval<TILE>(w0, epoch, sh.lb, ctrl, false, 0, excl, &err);
+                const uint32_t r = lb_reduce<TILE>(sh.lb, 0, excl);
                 if (r != LB_DONE) {
                     const int64_t base = (int64_t)p0.tile - 1 - (r == LB_NEXT ? (int64_t)lb_per(TILE) * TILE : 0);
                     excl = lookback_from<TILE>(status, base, pit.first_tile, epoch, sh.lb, ctrl, excl,
@@ -617,6 +662,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             np++;
             hb = hb == 2u ? 0u : hb + 1u;
         }
+        n = sh.next;
     }
     finish_launch(ctrl, epoch);
 }
