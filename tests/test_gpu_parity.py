@@ -217,3 +217,33 @@ def test_decode_with_and_without_prepass(gpu, orc, prepass):
             check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
     finally:
         gpu.set_decode_prepass(old)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "alternating", "sparse_dense"])
+def test_decode_many_tiles_per_workgroup(gpu, orc, kind):
+    """Decoding with one workgroup per CU, so each takes a dozen tiles or more:
+    dense streams use the pipelined decode (payload staged beside the previous
+    tile's in the LDS ring, wrapping round it; tiles too large to stage there
+    are loaded after the barrier), sparse ones the other decode.  Lossless
+    decodes equal the input; the lossy stream and decode equal the oracle's."""
+    import torch
+    rng = np.random.default_rng(41)
+    dims = (48, 1024, 1000)   # 750k blocks, 1465 tiles of 512
+    if kind == "uniform":          # 128 payload bytes per block: no tile fits beside another
+        vol = rng.integers(0, 65536, size=dims, dtype=np.uint16)
+    elif kind == "alternating":    # frames alternate incompressible and Poisson(0.3)
+        vol = rng.poisson(0.3, size=dims).astype(np.uint16)
+        vol[::2] = rng.integers(0, 65536, size=(dims[0] // 2,) + dims[1:], dtype=np.uint16)
+    else:                          # CSSI-like frames: a few KB per tile, the ring wraps
+        vol = synth("cssi", dims, seed=43)
+    try:
+        gpu.set_launch_shape(gpu.DECOMPRESS, 512, 1)
+        d = to_dev(vol)
+        out, n = gpu.compress(d, dims, (1, 1, 1, 1), 1.0)
+        dec = gpu.decompress(out, n, dims, (1, 1, 1, 1))
+        assert bool(torch.equal(dec, d))
+        if kind == "sparse_dense":
+            check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
+            check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+    finally:
+        gpu.set_launch_shape(gpu.DECOMPRESS, 0, 0)
