@@ -65,6 +65,8 @@ LSC_HD uint32_t bit_width(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz
 // Proof: f*v = (v/d)(1 + 2^-20)(1 + e), |e| <= 2^-23, so the product lies in
 // [v/d, v/d + (v/d) 2^-19).  v/d <= q + 1 - 1/d and (v/d) 2^-19 < 1/d for
 // v < 2^19, hence q <= f*v < q + 1.  Checked exhaustively on the CPU.
+// Pooled values floor((a + b) / 2) / d are floor((a + b) / 2d), the same
+// division of the pair sum (< 2^17) by 2d.
 // ---------------------------------------------------------------------------
 LSC_HD uint32_t quant(uint32_t v, float f) { return (uint32_t)((float)v * f); }
 
@@ -85,6 +87,7 @@ LSC_HD uint32_t fdiv(uint32_t n, FastDiv fd)
 struct Decide {
     uint32_t b[4];     // quantization bins
     float f[4];        // fl((1 + 2^-20) / b[k])
+    float f2[4];       // fl((1 + 2^-20) / (2 b[k])): pooled pair sums
     uint32_t zth;      // pool iff zero_count >= zth (65: never)
     uint32_t flags;    // DEC_*
     FastDiv fd[4];     // exact 32-bit division by b[k] (uint32 volumes)
@@ -105,12 +108,17 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
     uint32_t m1 = mx.x > mx.y ? mx.x : mx.y;
     if (p.flags & DEC_LOSSLESS) return (3u << 5) | bit_width(m1);
 
-    uint32_t pool = 0, m2 = 0;
+    uint32_t pool = 0, s2 = 0;
     if (p.flags & DEC_POOL) {
         // nonzero count (min(v,1) summed per half) and max pair sum
         u16x2 nz = {0, 0};
-        uint32_t s2 = 0;
-        const u16x2 one = {1, 1};
+        uint32_t one_w = 0x00010001u;
+#if defined(__HIP_DEVICE_COMPILE__)
+        // an opaque 1: with a visible constant the packed min becomes a
+        // compare, a select and a repack per half
+        asm("" : "+v"(one_w));
+#endif
+        const u16x2 one = as_u16x2(one_w);
 #pragma unroll
         for (int i = 0; i < 32; i++) {
             u16x2 v = as_u16x2(D[i]);
@@ -120,7 +128,6 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
         }
         uint32_t zc = 64u - ((uint32_t)nz.x + (uint32_t)nz.y);
         pool = zc >= p.zth;
-        m2 = s2 >> 1;   // max floor((a+b)/2) == floor(max(a+b)/2)
     }
 
     uint32_t bc = 3;
@@ -136,8 +143,10 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
         uint32_t m0 = (t <= 65535u - p.b[0]) ? t + p.b[0] : 0xffffffffu;
         bc = (uint32_t)(m0 >= p.b[1]) + (uint32_t)(m0 >= p.b[2]) + (uint32_t)(m0 >= p.b[3]);
     }
-    float f = bc == 0 ? p.f[0] : bc == 1 ? p.f[1] : bc == 2 ? p.f[2] : p.f[3];
-    uint32_t maxq = quant(pool ? m2 : m1, f);
+    // max floor(floor((a+b)/2) / bin) = floor(max(a+b) / 2bin)
+    const float *fs = pool ? p.f2 : p.f;
+    float f = bc == 0 ? fs[0] : bc == 1 ? fs[1] : bc == 2 ? fs[2] : fs[3];
+    uint32_t maxq = quant(pool ? s2 : m1, f);
     return (pool << 7) | (bc << 5) | bit_width(maxq);
 }
 
@@ -232,14 +241,12 @@ LSC_HD void quantize_plain(uint32_t D[32], float f)
     }
 }
 
-// Pooled: Q[t] = floor((v[2t] + v[2t+1]) / 2) / bin, t = 4r + c (pair c of row r).
-LSC_HD void quantize_pooled(const uint32_t D[32], float f, bool identity, uint32_t Q[32])
+// Pooled: Q[t] = floor((v[2t] + v[2t+1]) / 2) / bin = floor((v[2t] + v[2t+1]) / 2bin),
+// t = 4r + c (pair c of row r); f2 = the pooled factor of the bin.
+LSC_HD void quantize_pooled(const uint32_t D[32], float f2, uint32_t Q[32])
 {
 #pragma unroll
-    for (int t = 0; t < 32; t++) {
-        uint32_t a = ((D[t] & 0xffffu) + (D[t] >> 16)) >> 1;
-        Q[t] = identity ? a : quant(a, f);
-    }
+    for (int t = 0; t < 32; t++) Q[t] = quant((D[t] & 0xffffu) + (D[t] >> 16), f2);
 }
 
 // Pooled column words: C[u] byte m = low (hi = false) or high byte of Q[8m+u].
@@ -267,7 +274,7 @@ LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint
     const float f = bc == 0 ? p.f[0] : bc == 1 ? p.f[1] : bc == 2 ? p.f[2] : p.f[3];
     if (hdr & 0x80u) {
         uint32_t Q[32], C[8];
-        quantize_pooled(D, f, identity, Q);
+        quantize_pooled(D, bc == 0 ? p.f2[0] : bc == 1 ? p.f2[1] : bc == 2 ? p.f2[2] : p.f2[3], Q);
         pooled_columns(Q, C, false);
         const uint32_t nlo = rate < 8 ? rate : 8u;
         for (uint32_t i = 0; i < nlo; i++) out[i] = plane_from_columns(C, i);

@@ -25,6 +25,7 @@ inline Decide make_decide(const uint32_t bins[4], float p)
     for (int k = 0; k < 4; k++) {
         d.b[k] = bins[k];
         d.f[k] = (float)((1.0 + 0x1p-20) / (double)bins[k]);
+        d.f2[k] = (float)((1.0 + 0x1p-20) / (2.0 * (double)bins[k]));
         d.fd[k] = make_fastdiv(bins[k]);
     }
     // pool iff (float)zero_count / 64 > p, as in cuLSZ_kernel.cu:920-921

@@ -76,6 +76,8 @@ def block_host():
     lib.bh_fdiv.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     lib.bh_quant_mismatches.restype = ctypes.c_uint64
     lib.bh_quant_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    lib.bh_quant2_mismatches.restype = ctypes.c_uint64
+    lib.bh_quant2_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     lib.bh_zth.restype = ctypes.c_uint32
     lib.bh_zth.argtypes = [ctypes.c_float]
     return lib

@@ -95,6 +95,26 @@ uint64_t bh_quant_mismatches(uint32_t d_lo, uint32_t d_hi)
     return bad;
 }
 
+// Counts the divisors d in [d_lo, d_hi] for which the pooled division
+// q = trunc((float)s * f2_d) differs from s / 2d for some pair sum s < 2^17.
+// trunc(s * f2) is non-decreasing in s, so it equals s / 2d everywhere iff it
+// does at both ends of every step [2dk, 2d(k+1) - 1].
+uint64_t bh_quant2_mismatches(uint32_t d_lo, uint32_t d_hi)
+{
+    uint64_t bad = 0;
+    for (uint64_t d = d_lo; d <= d_hi; d++) {
+        const uint32_t b4[4] = {(uint32_t)d, (uint32_t)d, (uint32_t)d, (uint32_t)d};
+        const float f2 = make_decide(b4, 1.0f).f2[0];
+        const uint64_t step = 2 * d;
+        for (uint64_t lo = 0; lo < (1u << 17); lo += step) {
+            const uint64_t hi = lo + step - 1 < (1u << 17) - 1 ? lo + step - 1 : (1u << 17) - 1;
+            bad += quant((uint32_t)lo, f2) != lo / step;
+            bad += quant((uint32_t)hi, f2) != hi / step;
+        }
+    }
+    return bad;
+}
+
 uint32_t bh_zth(float p)
 {
     const uint32_t b4[4] = {1, 1, 1, 1};

@@ -100,6 +100,15 @@ def test_quantize_exhaustive(block_host):
     assert block_host.bh_quant_mismatches(4294967000, 4294967295) == 0
 
 
+def test_quantize_pooled_sums(block_host):
+    """Pooled values come from the pair sum s < 2^17: trunc((float)s * f2_d) == s // 2d
+    at both ends of every step, for every d <= 65536 and large d (the function is
+    monotone in s, so this covers every s)."""
+    assert block_host.bh_quant2_mismatches(1, 65536) == 0
+    assert block_host.bh_quant2_mismatches(65537, 70000) == 0
+    assert block_host.bh_quant2_mismatches(4294967000, 4294967295) == 0
+
+
 def _blocks_u32(rng, n):
     out = []
     for i in range(n):
