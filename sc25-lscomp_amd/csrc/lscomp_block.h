@@ -70,6 +70,26 @@ LSC_HD uint32_t bit_width(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz
 // ---------------------------------------------------------------------------
 LSC_HD uint32_t quant(uint32_t v, float f) { return (uint32_t)((float)v * f); }
 
+// x[k] for a per-block k in 0..3, as a select between values.  The empty asm
+// keeps them values: the compiler would otherwise load through a selected
+// address into the kernel's parameter block, which then has to live in
+// scratch memory.
+LSC_HD uint32_t pick4(const uint32_t x[4], uint32_t k)
+{
+    uint32_t a = x[0], b = x[1], c = x[2], d = x[3];
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+#endif
+    return k == 0 ? a : k == 1 ? b : k == 2 ? c : d;
+}
+
+LSC_HD float pick4(const float x[4], uint32_t k)
+{
+    uint32_t u[4];
+    for (int i = 0; i < 4; i++) u[i] = __builtin_bit_cast(uint32_t, x[i]);
+    return __builtin_bit_cast(float, pick4(u, k));
+}
+
 // Fast u32 division by an invariant divisor (round-up multiplier method,
 // Granlund & Montgomery 1994): q = (umulhi(n, m) + n) >> s.
 struct FastDiv {
@@ -79,6 +99,18 @@ LSC_HD uint32_t fdiv(uint32_t n, FastDiv fd)
 {
     uint32_t hi = (uint32_t)(((uint64_t)n * fd.m) >> 32);
     return (uint32_t)(((uint64_t)hi + n) >> fd.s);
+}
+
+// fd[k] for a per-block k (see pick4)
+LSC_HD FastDiv pick4(const FastDiv fd[4], uint32_t k)
+{
+    const uint32_t d[4] = {fd[0].d, fd[1].d, fd[2].d, fd[3].d}, m[4] = {fd[0].m, fd[1].m, fd[2].m, fd[3].m};
+    const uint32_t sh[4] = {fd[0].s, fd[1].s, fd[2].s, fd[3].s};
+    FastDiv r{};
+    r.d = pick4(d, k);
+    r.m = pick4(m, k);
+    r.s = pick4(sh, k);
+    return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -144,8 +176,7 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
         bc = (uint32_t)(m0 >= p.b[1]) + (uint32_t)(m0 >= p.b[2]) + (uint32_t)(m0 >= p.b[3]);
     }
     // max floor(floor((a+b)/2) / bin) = floor(max(a+b) / 2bin)
-    const float *fs = pool ? p.f2 : p.f;
-    float f = bc == 0 ? fs[0] : bc == 1 ? fs[1] : bc == 2 ? fs[2] : fs[3];
+    const float f = pool ? pick4(p.f2, bc) : pick4(p.f, bc);
     uint32_t maxq = quant(pool ? s2 : m1, f);
     return (pool << 7) | (bc << 5) | bit_width(maxq);
 }
@@ -270,11 +301,11 @@ LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint
     const uint32_t rate = hdr & 0x1fu;
     if (rate == 0) return 0;
     const uint32_t bc = (hdr >> 5) & 3u;
-    const bool identity = p.b[bc] == 1u;
-    const float f = bc == 0 ? p.f[0] : bc == 1 ? p.f[1] : bc == 2 ? p.f[2] : p.f[3];
+    const bool identity = pick4(p.b, bc) == 1u;
+    const float f = pick4(p.f, bc);
     if (hdr & 0x80u) {
         uint32_t Q[32], C[8];
-        quantize_pooled(D, bc == 0 ? p.f2[0] : bc == 1 ? p.f2[1] : bc == 2 ? p.f2[2] : p.f2[3], Q);
+        quantize_pooled(D, pick4(p.f2, bc), Q);
         pooled_columns(Q, C, false);
         const uint32_t nlo = rate < 8 ? rate : 8u;
         for (uint32_t i = 0; i < nlo; i++) out[i] = plane_from_columns(C, i);
@@ -307,7 +338,7 @@ LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint
 LSC_HD void decode_block(const uint32_t *in, uint32_t hdr, const uint32_t bins[4], uint32_t D[32])
 {
     const uint32_t rate = hdr & 0x1fu;
-    const uint32_t bin = bins[(hdr >> 5) & 3u];
+    const uint32_t bin = pick4(bins, (hdr >> 5) & 3u);
     if (hdr & 0x80u) {
         uint32_t CP[8] = {0, 0, 0, 0, 0, 0, 0, 0}, CPH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const uint32_t nlo = rate < 8 ? rate : 8u;
@@ -394,7 +425,7 @@ LSC_HD uint32_t block_header_u32(const uint32_t V[64], const Decide &p, bool *ra
         }
     }
     const uint32_t mx = pool ? m2 : m1;
-    const uint32_t maxq = p.b[bc] == 1u ? mx : fdiv(mx, p.fd[bc]);
+    const uint32_t maxq = pick4(p.b, bc) == 1u ? mx : fdiv(mx, pick4(p.fd, bc));
     const uint32_t rate = bit_width(maxq);
     *rate32 = rate > 31u;
     return (pool << 7) | (bc << 5) | (rate & 0x1fu);
@@ -407,8 +438,8 @@ LSC_HD uint32_t encode_block_u32(uint32_t V[64], uint32_t hdr, const Decide &p, 
     const uint32_t rate = hdr & 0x1fu;
     if (rate == 0) return 0;
     const uint32_t bc = (hdr >> 5) & 3u;
-    const bool identity = p.b[bc] == 1u;
-    const FastDiv fd = p.fd[bc];
+    const bool identity = pick4(p.b, bc) == 1u;
+    const FastDiv fd = pick4(p.fd, bc);
     if (hdr & 0x80u) {
         uint32_t Q[32], C[8];
 #pragma unroll
@@ -454,7 +485,7 @@ LSC_HD uint32_t encode_block_u32(uint32_t V[64], uint32_t hdr, const Decide &p, 
 LSC_HD void decode_block_u32(const uint32_t *in, uint32_t hdr, const uint32_t bins[4], uint32_t V[64])
 {
     const uint32_t rate = hdr & 0x1fu;
-    const uint32_t bin = bins[(hdr >> 5) & 3u];
+    const uint32_t bin = pick4(bins, (hdr >> 5) & 3u);
 #pragma unroll
     for (int e = 0; e < 64; e++) V[e] = 0;
     if (rate == 0) return;

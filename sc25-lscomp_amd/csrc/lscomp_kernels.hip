@@ -163,6 +163,123 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
     }
 }
 
+constexpr uint32_t kWordsU32 = 62;   // payload words of a uint32 block at most (31 planes x 8 bytes)
+
+__device__ __forceinline__ void load_block_u32(const Item &it, uint32_t bx, uint32_t by, uint32_t bz,
+                                               uint32_t V[64])
+{
+    const uint32_t y0 = by * 8, z0 = bz * 8;
+    const uint32_t *p = (const uint32_t *)it.vol_in + ((uint64_t)bx * it.y + y0) * it.z + z0;
+    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
+    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
+    if (ny == 8u && nz == 8u) {
+        // two 16-byte loads per row (dword alignment suffices)
+        uint4 t[16];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            t[2 * r] = *(const uint4 *)(p + (uint64_t)r * it.z);
+            t[2 * r + 1] = *(const uint4 *)(p + (uint64_t)r * it.z + 4);
+        }
+#pragma unroll
+        for (int h = 0; h < 16; h++) {
+            V[4 * h] = t[h].x; V[4 * h + 1] = t[h].y; V[4 * h + 2] = t[h].z; V[4 * h + 3] = t[h].w;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                V[8 * r + k] = ((uint32_t)r < ny && (uint32_t)k < nz) ? p[(uint64_t)r * it.z + k] : 0u;
+    }
+}
+
+__device__ __forceinline__ void store_block_u32(const Item &it, uint32_t bx, uint32_t by, uint32_t bz,
+                                                const uint32_t V[64])
+{
+    const uint32_t y0 = by * 8, z0 = bz * 8;
+    uint32_t *p = (uint32_t *)it.vol_out + ((uint64_t)bx * it.y + y0) * it.z + z0;
+    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
+    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
+    if (ny == 8u && nz == 8u) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            *(uint4 *)(p + (uint64_t)r * it.z) = make_uint4(V[8 * r], V[8 * r + 1], V[8 * r + 2], V[8 * r + 3]);
+            *(uint4 *)(p + (uint64_t)r * it.z + 4) =
+                make_uint4(V[8 * r + 4], V[8 * r + 5], V[8 * r + 6], V[8 * r + 7]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if ((uint32_t)r < ny && (uint32_t)k < nz) p[(uint64_t)r * it.z + k] = V[8 * r + k];
+    }
+}
+
+// What the tile kernels need of a value type: registers per block, payload
+// words per block at most, and the per-block load, header, encode, decode and
+// store.
+struct CodecU16 {
+    static constexpr int kRegs = 32;              // 64 values as packed pairs
+    static constexpr uint32_t kMaxWords = 32;     // 16 planes x 8 bytes
+    __device__ __forceinline__ static void load(const Item &it, uint32_t bx, uint32_t by, uint32_t bz, uint32_t R[kRegs])
+    {
+        load_block(it, bx, by, bz, R);
+    }
+    __device__ __forceinline__ static uint32_t header(const uint32_t R[kRegs], const Item &it, Ctrl *)
+    {
+        return block_header(R, it.dec);
+    }
+    __device__ __forceinline__ static void encode(uint32_t R[kRegs], uint32_t hdr, const Item &it, uint32_t *out)
+    {
+        encode_block(R, hdr, it.dec, out);
+    }
+    __device__ __forceinline__ static bool bad_header(uint32_t hdr) { return (hdr & 0x1fu) > 16u; }
+    __device__ __forceinline__ static void decode(const uint32_t *in, uint32_t hdr, const Item &it, uint32_t R[kRegs])
+    {
+        decode_block(in, hdr, it.dec.b, R);
+    }
+    __device__ __forceinline__ static void store(const Item &it, uint32_t bx, uint32_t by, uint32_t bz, const uint32_t R[kRegs])
+    {
+        store_block(it, bx, by, bz, R);
+    }
+};
+
+struct CodecU32 {
+    static constexpr int kRegs = 64;
+    static constexpr uint32_t kMaxWords = kWordsU32;
+    __device__ __forceinline__ static void load(const Item &it, uint32_t bx, uint32_t by, uint32_t bz, uint32_t R[kRegs])
+    {
+        load_block_u32(it, bx, by, bz, R);
+    }
+    // A block whose quantized maximum needs 32 bits is not representable: the
+    // launch is flagged and the block's rate cleared, so the stream stays
+    // consistent.
+    __device__ __forceinline__ static uint32_t header(const uint32_t R[kRegs], const Item &it, Ctrl *ctrl)
+    {
+        bool rate32 = false;
+        uint32_t hdr = block_header_u32(R, it.dec, &rate32);
+        if (rate32) {
+            __hip_atomic_fetch_or(&ctrl->error, ERRF_RANGE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            hdr &= 0xe0u;
+        }
+        return hdr;
+    }
+    __device__ __forceinline__ static void encode(uint32_t R[kRegs], uint32_t hdr, const Item &it, uint32_t *out)
+    {
+        encode_block_u32(R, hdr, it.dec, out);
+    }
+    __device__ __forceinline__ static bool bad_header(uint32_t) { return false; }   // every 5-bit rate decodes
+    __device__ __forceinline__ static void decode(const uint32_t *in, uint32_t hdr, const Item &it, uint32_t R[kRegs])
+    {
+        decode_block_u32(in, hdr, it.dec.b, R);
+    }
+    __device__ __forceinline__ static void store(const Item &it, uint32_t bx, uint32_t by, uint32_t bz, const uint32_t R[kRegs])
+    {
+        store_block_u32(it, bx, by, bz, R);
+    }
+};
+
 // Workgroup exclusive scan of per-lane word counts; returns the lane's offset,
 // sets *agg to the tile total.  Contains barriers.
 template <int TILE>
@@ -264,10 +381,10 @@ struct LbShared {
     uint32_t err[2];
 };
 
-template <int TILE>
+template <int TILE, uint32_t W = 32>
 struct TileShared {
     static constexpr int kWaves = TILE / 64;
-    alignas(16) uint32_t pay[TILE * 32 + 8];   // payload staging (128 B per block max) + shift slack
+    alignas(16) uint32_t pay[TILE * W + 8];   // payload staging (W words per block max) + shift slack
     uint32_t hdr[3][TILE / 4];     // header bytes, one buffer per pipeline stage
     uint32_t wsum[kWaves];
     LbShared<kWaves * lb_per(TILE)> lb;
@@ -486,9 +603,9 @@ __device__ __forceinline__ uint32_t ring_place(uint32_t len, uint32_t nlive, con
     const bool ua = nlive >= 1 && a.agg != 0, ub = nlive == 2 && b.agg != 0;
     if (!ua && !ub) return 0;
     if (ua != ub) {   // one run
-        const PendTile &r = ua ? a : b;
-        if (r.start + r.agg + len <= ring) return r.start + r.agg;
-        return len <= r.start ? 0u : ~0u;
+        const uint32_t start = ua ? a.start : b.start, end = ua ? a.start + a.agg : b.start + b.agg;
+        if (end + len <= ring) return end;
+        return len <= start ? 0u : ~0u;
     }
     const uint32_t tail = b.start + b.agg;
     if (b.start > a.start) {   // in address order
@@ -544,13 +661,13 @@ __device__ __forceinline__ void store_tile(const Item &it, uint32_t tl, uint32_t
 // without an inclusive prefix belongs to a workgroup that, whatever stage the
 // tile is in, either resolves it (all its predecessors are inclusive) or
 // finishes lower pending tiles of its own first, so it always advances.
-template <int TILE, bool kTable>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(
-    const Item single, const Item *__restrict__ items, const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
-    const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
+template <class C, int TILE, bool kTable>
+__device__ __forceinline__ void compress_tiles(const Item single, const Item *__restrict__ items,
+                                               const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
+                                               const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
 {
-    constexpr uint32_t kRing = (uint32_t)TILE * 32u;   // staging words: one worst-case tile
-    __shared__ TileShared<TILE> sh;
+    constexpr uint32_t kRing = (uint32_t)TILE * C::kMaxWords;   // staging words: one worst-case tile
+    __shared__ TileShared<TILE, C::kMaxWords> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid == 0) {
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -569,7 +686,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         const bool resolve = np == 2 || (!have_n && np != 0);
 
         // 1. loads of n, probe of p0, next id
-        uint32_t D[32];
+        uint32_t D[C::kRegs];
         uint32_t n_item = 0, n_tl = 0;
         bool nvalid = false;
         if (have_n) {
@@ -581,7 +698,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (nvalid) {
                 uint32_t bx, by, bz;
                 block_coords(nit, b, bx, by, bz);
-                load_block(nit, bx, by, bz, D);
+                C::load(nit, bx, by, bz, D);
             }
         }
         const bool p_lb = resolve && p0.tl != 0;   // p0 needs a look-back
@@ -597,7 +714,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         uint32_t n_hdr = 0, n_words = 0;
         if (have_n) {
             if (nvalid) {
-                n_hdr = block_header(D, item_at<kTable>(single, items, n_item).dec);
+                n_hdr = C::header(D, item_at<kTable>(single, items, n_item), ctrl);
                 n_words = header_payload_bytes(n_hdr) >> 2;
             }
             ((uint8_t *)sh.hdr[hb])[tid] = (uint8_t)n_hdr;
@@ -619,7 +736,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         // 3. pack n behind the pending tiles if it fits in the staging ring
         uint32_t n_start = have_n ? ring_place(n_agg, np, p0, p1, kRing) : ~0u;
         if (n_start != ~0u && n_words)
-            encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
+            C::encode(D, n_hdr, item_at<kTable>(single, items, n_item), &sh.pay[n_start + n_off]);
 
         // 4. finish p0
         if (resolve) {
@@ -656,7 +773,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                 np--;
                 __syncthreads();
             }
-            if (n_words) encode_block(D, n_hdr, item_at<kTable>(single, items, n_item).dec, &sh.pay[n_start + n_off]);
+            if (n_words) C::encode(D, n_hdr, item_at<kTable>(single, items, n_item), &sh.pay[n_start + n_off]);
             __syncthreads();
         }
         if (have_n) {
@@ -668,6 +785,14 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         n = sh.next;
     }
     finish_launch(ctrl, epoch);
+}
+
+template <int TILE, bool kTable>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(
+    const Item single, const Item *__restrict__ items, const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
+    const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
+{
+    compress_tiles<CodecU16, TILE, kTable>(single, items, tile_items, total_tiles, d_total_tiles, ctrl, status);
 }
 
 // ---------------------------------------------------------------------------
@@ -793,12 +918,12 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // payload-dense streams (see use_pipe_decode); on sparse ones, where a tile's
 // payload is a few hundred bytes, the decode with the payload loaded in its own
 // iteration was 2-5 % faster in timings.
-template <int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_pipe_kernel(
-    const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
+template <class C, int TILE>
+__device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
 {
-    constexpr uint32_t kRing = (uint32_t)TILE * 32u + 8u;   // staging words (sh.pay)
-    __shared__ TileShared<TILE> sh;
+    constexpr uint32_t kMax = (uint32_t)TILE * C::kMaxWords;   // payload words of a tile at most
+    constexpr uint32_t kRing = kMax + 8u;                     // staging words (sh.pay)
+    __shared__ TileShared<TILE, C::kMaxWords> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t ntot = it.ntiles;
     uint32_t pend = 0;   // thread 0: the id of the tile two ahead of the next one
@@ -826,7 +951,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     uint64_t g0 = g0_of(t), g1 = g0_of(t1);
     uint32_t agg = 0, t_start = 0;
     uint32_t off = tile_scan<TILE>(header_payload_bytes(hdr) >> 2, tid, sh.wsum, &agg);
-    bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
+    bool fits = agg <= kMax && g0 + 4ull * agg <= it.s_len;
     if (t < ntot && fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
     __syncthreads();
 
@@ -853,7 +978,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         //    wait for the payload in step 4 then covers all of them)
         const uint32_t hdr2 = hdr_of(t2);
         const uint64_t g2 = g0_of(t2);
-        const bool fits1 = t1 < ntot && agg1 <= (uint32_t)TILE * 32u && g1 + 4ull * agg1 <= it.s_len;
+        const bool fits1 = t1 < ntot && agg1 <= kMax && g1 + 4ull * agg1 <= it.s_len;
         const uint64_t a1 = g1 & ~15ull;
         const uint32_t nchunk1 = fits1 ? (((uint32_t)(g1 - a1) >> 2) + agg1 + 3) >> 2 : 0u;
         const uint32_t len0 = fits ? 4u * ((((uint32_t)(g0 & 15) >> 2) + agg + 3) >> 2) : 0u;
@@ -873,20 +998,20 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         // 3. decode t
         const uint32_t b = t * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
-        const bool bad = (hdr & 0x1fu) > 16u || !fits;
-        uint32_t D[32];
+        const bool bad = C::bad_header(hdr) || !fits;
+        uint32_t D[C::kRegs];
         if (valid && !bad && (hdr & 0x1fu)) {
-            decode_block(&sh.pay[t_start + ((uint32_t)(g0 & 15) >> 2) + off], hdr, it.dec.b, D);
+            C::decode(&sh.pay[t_start + ((uint32_t)(g0 & 15) >> 2) + off], hdr, it, D);
         } else {
 #pragma unroll
-            for (int i = 0; i < 32; i++) D[i] = 0;
+            for (int i = 0; i < C::kRegs; i++) D[i] = 0;
         }
 
         // 4. t's stores, then t1's payload into the ring
         if (valid) {
             uint32_t bx, by, bz;
             block_coords(it, b, bx, by, bz);
-            store_block(it, bx, by, bz, D);
+            C::store(it, bx, by, bz, D);
             if (bad) __hip_atomic_fetch_or(&ctrl->error, ERRF_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (t1_start != ~0u) {
@@ -913,6 +1038,13 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         t1 = t2; hdr1 = hdr2; g1 = g2;
     }
     finish_launch(ctrl, epoch);
+}
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_pipe_kernel(
+    const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
+{
+    decompress_pipe_tiles<CodecU16, TILE>(it, ctrl, tile_off);
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,186 +1190,27 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
 
 // ---------------------------------------------------------------------------
 // uint32 volumes (replace cuLSZ_{compression,decompression}_kernel_uint32_bsize64,
-// lsCOMP/src/cuLSZ_kernel.cu:4-824).  Same tile scheme as uint16 with one block
-// of 64 uint32 values per lane: dynamic tile ids, header bytes, workgroup
-// scan, aggregate published before the look-back, payload packed in LDS and
-// stored with 16-byte stores; decompression uses the tile offsets of
-// offsets_kernel (the header format is the same).  A block whose quantized
-// maximum needs 32 bits flags the launch (ERRF_RANGE).
+// lsCOMP/src/cuLSZ_kernel.cu:4-824): the uint16 compression and pipelined
+// decompression with the uint32 codec (CodecU32: one block of 64 uint32 values
+// per lane in 64 VGPRs, up to 62 payload words).  Tile 256: 62 KB of payload
+// staging per workgroup, 2 workgroups per CU (2 waves per SIMD, so up to 256
+// VGPRs).  The pipelined decode beat staging each payload in its own
+// iteration on sparse uint32 streams as well (lossy CSSI: 1.49 against
+// 1.68 ms), so it is the only uint32 decode.  A block whose quantized maximum
+// needs 32 bits flags the launch (ERRF_RANGE).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kWordsU32 = 62;   // payload words of a block at most (31 planes x 8 bytes)
-// LDS (62 KB of payload staging per workgroup of 256) allows 2 workgroups per
-// CU, i.e. 2 waves per SIMD, so the kernels may use up to 256 VGPRs.
-
-template <int TILE>
-struct TileShared32 {
-    alignas(16) uint32_t pay[TILE * kWordsU32 + 8];
-    uint32_t hdr[TILE / 4];
-    uint32_t wsum[TILE / 64];
-    LbShared<TILE / 64 * lb_per(TILE)> lb;
-    uint32_t tile, epoch;
-};
-
-__device__ __forceinline__ void load_block_u32(const Item &it, uint32_t bx, uint32_t by, uint32_t bz,
-                                               uint32_t V[64])
-{
-    const uint32_t y0 = by * 8, z0 = bz * 8;
-    const uint32_t *p = (const uint32_t *)it.vol_in + ((uint64_t)bx * it.y + y0) * it.z + z0;
-    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
-    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
-    if (ny == 8u && nz == 8u) {
-        // two 16-byte loads per row (dword alignment suffices)
-        uint4 t[16];
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            t[2 * r] = *(const uint4 *)(p + (uint64_t)r * it.z);
-            t[2 * r + 1] = *(const uint4 *)(p + (uint64_t)r * it.z + 4);
-        }
-#pragma unroll
-        for (int h = 0; h < 16; h++) {
-            V[4 * h] = t[h].x; V[4 * h + 1] = t[h].y; V[4 * h + 2] = t[h].z; V[4 * h + 3] = t[h].w;
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < 8; r++)
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                V[8 * r + k] = ((uint32_t)r < ny && (uint32_t)k < nz) ? p[(uint64_t)r * it.z + k] : 0u;
-    }
-}
-
-__device__ __forceinline__ void store_block_u32(const Item &it, uint32_t bx, uint32_t by, uint32_t bz,
-                                                const uint32_t V[64])
-{
-    const uint32_t y0 = by * 8, z0 = bz * 8;
-    uint32_t *p = (uint32_t *)it.vol_out + ((uint64_t)bx * it.y + y0) * it.z + z0;
-    const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
-    const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
-    if (ny == 8u && nz == 8u) {
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            *(uint4 *)(p + (uint64_t)r * it.z) = make_uint4(V[8 * r], V[8 * r + 1], V[8 * r + 2], V[8 * r + 3]);
-            *(uint4 *)(p + (uint64_t)r * it.z + 4) =
-                make_uint4(V[8 * r + 4], V[8 * r + 5], V[8 * r + 6], V[8 * r + 7]);
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < 8; r++)
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                if ((uint32_t)r < ny && (uint32_t)k < nz) p[(uint64_t)r * it.z + k] = V[8 * r + k];
-    }
-}
-
 template <int TILE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void compress_u32_kernel(
     const Item it, Ctrl *ctrl, uint64_t *status)
 {
-    __shared__ TileShared32<TILE> sh;
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) {
-        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const uint32_t epoch = sh.epoch, ntot = it.ntiles;
-    for (;;) {
-        const uint32_t tile = sh.tile;
-        if (tile >= ntot) break;
-        const uint32_t b = tile * (uint32_t)TILE + tid;
-        const bool valid = b < it.nblocks;
-        uint32_t V[64];
-        uint32_t hdr = 0;
-        if (valid) {
-            uint32_t bx, by, bz;
-            block_coords(it, b, bx, by, bz);
-            load_block_u32(it, bx, by, bz, V);
-            bool rate32 = false;
-            hdr = block_header_u32(V, it.dec, &rate32);
-            if (rate32) {   // not representable: flag the launch, keep the stream consistent
-                __hip_atomic_fetch_or(&ctrl->error, ERRF_RANGE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                hdr &= 0xe0u;
-            }
-        }
-        const uint32_t words = header_payload_bytes(hdr) >> 2;
-        ((uint8_t *)sh.hdr)[tid] = (uint8_t)hdr;
-        uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
-        if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tile == 0 ? LB_INCL : LB_AGG, agg));
-        if (words) encode_block_u32(V, hdr, it.dec, &sh.pay[off]);
-        uint64_t excl = 0;
-        if (tile != 0) {
-            excl = lookback_tile<TILE>(status, tile, 0, epoch, sh.lb, ctrl);
-            if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
-        }
-        __syncthreads();   // payload and header bytes are in LDS
-        const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
-        if (tid < (uint32_t)TILE / 16) {
-            const uint64_t o = (uint64_t)tile * TILE + 16ull * tid;
-            if (o + 16 <= it.hdr_bytes) {
-                *(uint4 *)(it.s_out + o) = make_uint4(sh.hdr[4 * tid], sh.hdr[4 * tid + 1], sh.hdr[4 * tid + 2],
-                                                      sh.hdr[4 * tid + 3]);
-            } else {
-#pragma unroll
-                for (uint32_t k = 0; k < 4; k++)
-                    if (o + 4 * k < it.hdr_bytes) *(uint32_t *)(it.s_out + o + 4 * k) = sh.hdr[4 * tid + k];
-            }
-        }
-        if (g0 + 4ull * agg <= it.s_len) store_payload<TILE>(sh.pay, agg, it.s_out + g0, tid);
-        if (tid == 0 && tile == ntot - 1 && it.result) *it.result = g0 + 4ull * agg;
-        __syncthreads();   // LDS is free again
-        if (tid == 0)      // no waits remain in this iteration: take the next id
-            sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-    }
-    finish_launch(ctrl, epoch);
+    compress_tiles<CodecU32, TILE, false>(it, nullptr, nullptr, it.ntiles, nullptr, ctrl, status);
 }
 
 template <int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void decompress_u32_kernel(
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void decompress_u32_pipe_kernel(
     const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
 {
-    __shared__ TileShared32<TILE> sh;
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) {
-        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const uint32_t epoch = sh.epoch, ntot = it.ntiles;
-    for (;;) {
-        const uint32_t tile = sh.tile;
-        if (tile >= ntot) break;
-        const uint32_t b = tile * (uint32_t)TILE + tid;
-        const bool valid = b < it.nblocks;
-        const uint32_t hdr = valid ? (uint32_t)it.s_in[b] : 0u;
-        const uint64_t excl = tile_off[tile];
-        uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(header_payload_bytes(hdr) >> 2, tid, sh.wsum, &agg);
-        const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
-        const bool fits = agg <= (uint32_t)TILE * kWordsU32 && g0 + 4ull * agg <= it.s_len;
-        const uint32_t shift = (uint32_t)(g0 & 15) >> 2;
-        if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
-        __syncthreads();
-        if (valid) {
-            uint32_t V[64];
-            if (fits) {
-                decode_block_u32(&sh.pay[shift + off], hdr, it.dec.b, V);
-            } else {
-#pragma unroll
-                for (int i = 0; i < 64; i++) V[i] = 0;
-                __hip_atomic_fetch_or(&ctrl->error, ERRF_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            uint32_t bx, by, bz;
-            block_coords(it, b, bx, by, bz);
-            store_block_u32(it, bx, by, bz, V);
-        }
-        __syncthreads();   // LDS is free again
-        if (tid == 0)
-            sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-    }
-    finish_launch(ctrl, epoch);
+    decompress_pipe_tiles<CodecU32, TILE>(it, ctrl, tile_off);
 }
 
 hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,
@@ -1252,7 +1225,7 @@ hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, 
         const uint32_t nchunks = (it.ntiles + 63) / 64;
         const uint32_t g = nchunks < cfg.grid ? nchunks : cfg.grid;
         hipLaunchKernelGGL((offsets_kernel<T>), dim3(g), dim3(256), 0, stream, it, ctrl, status, cfg.tile_off);
-        hipLaunchKernelGGL((decompress_u32_kernel<T>), grid, block, 0, stream, it, ctrl,
+        hipLaunchKernelGGL((decompress_u32_pipe_kernel<T>), grid, block, 0, stream, it, ctrl,
                            (const uint64_t *)cfg.tile_off);
     }
     return hipGetLastError();
@@ -1320,7 +1293,7 @@ int workgroups_per_cu_u32(bool compress)
     int per_cu = 0;
     const hipError_t e = compress
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, compress_u32_kernel<T>, T, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_u32_kernel<T>, T, 0);
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_u32_pipe_kernel<T>, T, 0);
     return (e == hipSuccess && per_cu > 0) ? per_cu : 1;
 }
 
