@@ -1050,8 +1050,9 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // ---------------------------------------------------------------------------
 // Decode pre-pass: payload offset of every tile of a single item, from the
 // header bytes alone.  A workgroup of 256 threads takes a chunk of 64 tiles
-// (4 threads per tile, TILE/4 header bytes each), reduces sizes per tile,
-// scans them, and chains chunks with the same look-back.  tile_off[t] is the
+// (4 threads per tile, TILE/4 header bytes each, read as 16-byte pieces so
+// that a wave's load covers 64 contiguous bytes of each of 16 tiles), reduces
+// sizes per tile, scans them, and chains chunks with the same look-back.  tile_off[t] is the
 // word offset of tile t's payload after the header section; tile_off[ntiles]
 // is the total.
 // ---------------------------------------------------------------------------
@@ -1087,21 +1088,33 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
     for (;;) {
         const uint32_t chunk = sh.tile;
         if (chunk >= nchunks) break;
+        // thread (j, q) sums the 16-byte pieces q, q+4, q+8, ... of tile j: each
+        // load instruction of a wave reads 64 contiguous bytes of 16 tiles
         const uint32_t j = tid >> 2, q = tid & 3u;
         const uint32_t tile = chunk * 64 + j;
-        const uint64_t first = (uint64_t)tile * TILE + q * kBytes;   // first header byte of this thread
+        const uint64_t tbase = (uint64_t)tile * TILE;   // the tile's first header byte
         uint32_t sum = 0;
-        if (first < it.nblocks) {
-            const uint32_t *src = (const uint32_t *)(it.s_in + first);
-            const uint32_t nvalid = (uint32_t)((it.nblocks - first) < kBytes ? (it.nblocks - first) : kBytes);
 #pragma unroll
-            for (uint32_t k = 0; k < kBytes / 4; k++) {
-                if (4 * k >= nvalid) break;   // never read past the header section
-                uint32_t x = src[k];
-                const uint32_t keep = nvalid - 4 * k;
-                if (keep < 4) x &= 0xffffffffu >> (32 - 8 * keep);   // clip bytes past the last block
-                sum += words4(x);
+        for (uint32_t k = 0; k < kBytes / 16; k++) {
+            const uint64_t off = tbase + 16ull * (4 * k + q);
+            if (off >= it.nblocks) break;
+            uint4 x;
+            if (off + 16 <= it.nblocks) {
+                x = *(const uint4 *)(it.s_in + off);
+            } else {   // the section's last piece: dwords below R, bytes past the last block cleared
+                const uint32_t *s32 = (const uint32_t *)(it.s_in + off);
+                uint32_t t[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const uint64_t at = off + 4u * (uint32_t)d;
+                    if (at < it.nblocks) {
+                        const uint64_t keep = it.nblocks - at;
+                        t[d] = keep < 4 ? s32[d] & (0xffffffffu >> (32 - 8 * (uint32_t)keep)) : s32[d];
+                    }
+                }
+                x = make_uint4(t[0], t[1], t[2], t[3]);
             }
+            sum += words4(x.x) + words4(x.y) + words4(x.z) + words4(x.w);
         }
         sum += __shfl_xor(sum, 1, 64);
         sum += __shfl_xor(sum, 2, 64);
