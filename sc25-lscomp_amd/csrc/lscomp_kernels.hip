@@ -385,10 +385,12 @@ template <int TILE, uint32_t W = 32>
 struct TileShared {
     static constexpr int kWaves = TILE / 64;
     alignas(16) uint32_t pay[TILE * W + 8];   // payload staging (W words per block max) + shift slack
-    uint32_t hdr[3][TILE / 4];     // header bytes, one buffer per pipeline stage
-    uint32_t wsum[kWaves];
-    LbShared<kWaves * lb_per(TILE)> lb;
+    uint32_t hdr[4][TILE / 4];     // header bytes, one buffer per tile in flight (compression)
+    uint32_t wsum[2][kWaves];      // wave totals of a scan, by iteration parity
+    LbShared<kWaves * lb_per(TILE)> lb;    // look-back rounds
+    LbShared<kWaves * lb_per(TILE)> lbf;   // compression: first round, by iteration parity
     uint32_t tile, next, epoch, ntot;
+    uint32_t next2[2];             // compression: next tile id, by iteration parity
 };
 
 // One probe round: this thread's words of the predecessors at distances
@@ -655,8 +657,12 @@ __device__ __forceinline__ void store_tile(const Item &it, uint32_t tl, uint32_t
 // By the time p is resolved its predecessors have had a whole iteration to
 // publish their aggregates, so the look-back rarely waits, and a workgroup
 // that does wait does not hold up the tiles behind its reserved id for long.
-// (Timed alternatives: issuing the next loads before the closing barrier, and
-// storing p before packing n, were both slower.)
+// The scan barrier is the only one an iteration needs: what is written before
+// it (wave totals, the first look-back round, the next id) has one LDS buffer
+// per iteration parity; header bytes have four buffers (a tile's buffer is
+// rewritten four tiles later); a ring region is rewritten only by a later
+// iteration's pack, after its scan barrier.  (Timed alternatives: issuing the
+// next loads early, and storing p before packing n, were both slower.)
 // Forward progress: a look-back waits only for lower tile ids; the lowest tile
 // without an inclusive prefix belongs to a workgroup that, whatever stage the
 // tile is in, either resolves it (all its predecessors are inclusive) or
@@ -667,6 +673,12 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
                                                const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
 {
     constexpr uint32_t kRing = (uint32_t)TILE * C::kMaxWords;   // staging words: one worst-case tile
+    // An iteration whose resolved tile stored at least this many payload words
+    // ends with a barrier.  Timed on the CSSI volume: without any closing
+    // barrier lossy compression (~0.1 KB per tile) ran 5 % faster and lossless
+    // (~8.6 KB per tile) 4 % slower; with it after large tiles only, both were
+    // at their best.
+    constexpr uint32_t kSyncWords = 512;
     __shared__ TileShared<TILE, C::kMaxWords> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid == 0) {
@@ -679,8 +691,8 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
 
     uint32_t np = 0, hb = 0;   // pending tiles (p0 oldest), header buffer of the next tile
     PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
-    uint32_t n = sh.next;
-    for (;;) {
+    uint32_t n = sh.next, par = 0;   // par: iteration parity (LDS buffers written before the scan barrier)
+    for (;; par ^= 1u) {
         const bool have_n = n < ntot;
         if (!have_n && np == 0) break;
         const bool resolve = np == 2 || (!have_n && np != 0);
@@ -720,13 +732,15 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
             ((uint8_t *)sh.hdr[hb])[tid] = (uint8_t)n_hdr;
         }
         const uint32_t incl = wave_incl_scan(n_words, lane);
-        if (lane == 63) sh.wsum[wave] = incl;
-        if (p_lb) lb_slices<TILE>(w0, epoch, sh.lb, 0);
+        if (lane == 63) sh.wsum[par][wave] = incl;
+        if (p_lb) lb_slices<TILE>(w0, epoch, sh.lbf, par);
+        if (tid == 0) sh.next2[par] = next_id;
         __syncthreads();
+        const uint32_t n_next = sh.next2[par];
         uint32_t before = 0, n_agg = 0;
 #pragma unroll
         for (int v = 0; v < TILE / 64; v++) {
-            const uint32_t x = sh.wsum[v];
+            const uint32_t x = sh.wsum[par][v];
             before += (v < (int)wave) ? x : 0u;
             n_agg += x;
         }
@@ -743,22 +757,22 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
             const Item &pit = item_at<kTable>(single, items, p0.item);
             uint64_t excl = 0;
             if (p_lb) {
-                const uint32_t r = lb_reduce<TILE>(sh.lb, 0, excl);
+                const uint32_t r = lb_reduce<TILE>(sh.lbf, par, excl);
                 if (r != LB_DONE) {
                     const int64_t base = (int64_t)p0.tile - 1 - (r == LB_NEXT ? (int64_t)lb_per(TILE) * TILE : 0);
                     excl = lookback_from<TILE>(status, base, pit.first_tile, epoch, sh.lb, ctrl, excl,
-                                               r == LB_RETRY ? 1u : 0u, 1u);
+                                               r == LB_RETRY ? 1u : 0u, 0u);
                 }
                 if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
             }
             store_tile<TILE>(pit, p0.tl, p0.agg, sh.hdr[p0.hb], sh.pay + p0.start, excl, tid);
+            if (p0.agg >= kSyncWords) __syncthreads();
             p0 = p1;
             np--;
         }
-        if (tid == 0) sh.next = next_id;
-        __syncthreads();   // p0's payload has left LDS; n's packed payload and sh.next are visible
         if (have_n && n_start == ~0u) {
             // n did not fit: finish pending tiles (oldest first) until it does
+            __syncthreads();   // p0's payload has left LDS
             for (;;) {
                 n_start = ring_place(n_agg, np, p0, p1, kRing);
                 if (n_start != ~0u) break;
@@ -780,9 +794,9 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
             const PendTile t = {n, n_tl, n_item, n_agg, n_start, hb};
             if (np == 0) p0 = t; else p1 = t;
             np++;
-            hb = hb == 2u ? 0u : hb + 1u;
+            hb = (hb + 1u) & 3u;
         }
-        n = sh.next;
+        n = n_next;
     }
     finish_launch(ctrl, epoch);
 }
@@ -849,7 +863,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         bool bad = (hdr & 0x1fu) > 16u;
 
         uint32_t agg;
-        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum, &agg);
+        const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum[0], &agg);
         if (!tile_off) {
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
             excl = 0;
@@ -950,7 +964,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
     uint32_t hdr = hdr_of(t), hdr1 = hdr_of(t1);
     uint64_t g0 = g0_of(t), g1 = g0_of(t1);
     uint32_t agg = 0, t_start = 0;
-    uint32_t off = tile_scan<TILE>(header_payload_bytes(hdr) >> 2, tid, sh.wsum, &agg);
+    uint32_t off = tile_scan<TILE>(header_payload_bytes(hdr) >> 2, tid, sh.wsum[0], &agg);
     bool fits = agg <= kMax && g0 + 4ull * agg <= it.s_len;
     if (t < ntot && fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
     __syncthreads();
@@ -959,14 +973,14 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
         // 1. scan of t1's headers; t2's id
         const uint32_t words1 = header_payload_bytes(hdr1) >> 2;
         const uint32_t incl = wave_incl_scan(words1, lane);
-        if (lane == 63) sh.wsum[wave] = incl;
+        if (lane == 63) sh.wsum[0][wave] = incl;
         if (tid == 0) sh.next = pend;
         __syncthreads();
         const uint32_t t2 = __builtin_amdgcn_readfirstlane(sh.next);
         uint32_t before = 0, agg1 = 0;
 #pragma unroll
         for (int v = 0; v < TILE / 64; v++) {
-            const uint32_t x = sh.wsum[v];
+            const uint32_t x = sh.wsum[0][v];
             before += (v < (int)wave) ? x : 0u;
             agg1 += x;
         }
