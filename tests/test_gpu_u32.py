@@ -119,3 +119,27 @@ def test_u32_misaligned_and_capacity(gpu, orc):
     assert e.value.status == gpu.ERR_CORRUPT
     dec = gpu.decompress_u32(out, refn, vol.shape).cpu().numpy().view(np.uint32)
     assert np.array_equal(dec, vol)
+
+
+@pytest.mark.parametrize("kind", ["rate24", "alternating", "u16x4099"])
+def test_u32_many_tiles_per_workgroup(gpu, orc, kind):
+    """4096 tiles of 256 blocks over 512 workgroups (8 each), lossless: payloads
+    too large to stage beside the previous tile (24-bit values), alternating
+    dense and sparse frames, and CSSI-like values scaled up (the staging ring
+    wraps).  The decode equals the input; a slab of frames also equals the
+    oracle's stream byte for byte."""
+    import torch
+    rng = np.random.default_rng(51)
+    dims = (64, 1024, 1024)
+    if kind == "rate24":
+        vol = rng.integers(0, 1 << 24, size=dims, dtype=np.uint32)
+    elif kind == "alternating":
+        vol = rng.poisson(0.3, size=dims).astype(np.uint32)
+        vol[::2] = rng.integers(0, 1 << 20, size=(dims[0] // 2,) + dims[1:], dtype=np.uint32)
+    else:
+        vol = synth("cssi", dims, seed=53).astype(np.uint32) * 4099
+    d = to_dev32(vol)
+    out, n = gpu.compress_u32(d, dims, (1, 1, 1, 1), 1.0)
+    dec = gpu.decompress_u32(out, n, dims, (1, 1, 1, 1))
+    assert bool(torch.equal(dec, d))
+    check_u32(gpu, orc, np.ascontiguousarray(vol[:4]), (1, 1, 1, 1), 1.0)
