@@ -112,15 +112,48 @@ __device__ __forceinline__ void load_block(const Item &it, uint32_t bx, uint32_t
             D[4 * r + 3] = keep > 3 ? t[r].w : 0u;
         }
     } else {
+        // Rows only 2-byte aligned (z odd, or a volume starting at 2 mod 4).  A
+        // row at a 4-byte boundary loads 16 bytes; a row at 2 mod 4 loads the 20
+        // bytes from the dword boundary one element before it and shifts them
+        // down by one element (v_alignbit_b32).  A block at a row end (nz < 8)
+        // does the same while the bytes lie inside the volume and clears the
+        // elements past z; rows whose bytes would leave the volume (its first
+        // and last rows) load by elements.
+        const uint16_t *vol_end = it.vol_in + (uint64_t)it.x * it.y * it.z;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const bool rv = (uint32_t)r < ny;
             const uint16_t *q = p + (uint64_t)r * it.z;
+            bool got = false;
+            uint32_t w[4];
+            if (rv && ((uintptr_t)q & 2u) == 0 && q + 8 <= vol_end) {
+                const uint4 t = *(const uint4 *)q;
+                w[0] = t.x;
+                w[1] = t.y;
+                w[2] = t.z;
+                w[3] = t.w;
+                got = true;
+            } else if (rv && q > it.vol_in && q + 9 <= vol_end) {
+                const uint32_t *a = (const uint32_t *)(q - 1);   // dword boundary
+                const uint4 t = *(const uint4 *)a;
+                const uint32_t e = a[4];
+                w[0] = __builtin_amdgcn_alignbit(t.y, t.x, 16);
+                w[1] = __builtin_amdgcn_alignbit(t.z, t.y, 16);
+                w[2] = __builtin_amdgcn_alignbit(t.w, t.z, 16);
+                w[3] = __builtin_amdgcn_alignbit(e, t.w, 16);
+                got = true;
+            }
+            if (got) {
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const uint32_t lo = (rv && 2u * c < nz) ? q[2 * c] : 0u;
-                const uint32_t hi = (rv && 2u * c + 1 < nz) ? q[2 * c + 1] : 0u;
-                D[4 * r + c] = lo | (hi << 16);
+                for (int c = 0; c < 4; c++)   // elements 2c, 2c+1; clear those at or past nz
+                    D[4 * r + c] = 2u * c + 1 < nz ? w[c] : 2u * c < nz ? (w[c] & 0xffffu) : 0u;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const uint32_t lo = (rv && 2u * c < nz) ? q[2 * c] : 0u;
+                    const uint32_t hi = (rv && 2u * c + 1 < nz) ? q[2 * c + 1] : 0u;
+                    D[4 * r + c] = lo | (hi << 16);
+                }
             }
         }
     }
@@ -150,14 +183,43 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
             }
         }
     } else {
+        // Rows only 2-byte aligned: a row at a 4-byte boundary stores 16 bytes, a
+        // row at 2 mod 4 its first element, the next pairs as dwords and its last
+        // element; a block at a row end stores its nz elements the same way
+        // (nothing outside the block is written).
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             if ((uint32_t)r >= ny) break;
             uint16_t *q = p + (uint64_t)r * it.z;
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                if (2u * c < nz) q[2 * c] = (uint16_t)(D[4 * r + c] & 0xffffu);
-                if (2u * c + 1 < nz) q[2 * c + 1] = (uint16_t)(D[4 * r + c] >> 16);
+            const uint32_t d0 = D[4 * r], d1 = D[4 * r + 1], d2 = D[4 * r + 2], d3 = D[4 * r + 3];
+            if (nz == 8u && ((uintptr_t)q & 2u) == 0) {
+                *(uint4 *)q = make_uint4(d0, d1, d2, d3);
+            } else if (nz == 8u) {
+                q[0] = (uint16_t)(d0 & 0xffffu);
+                uint32_t *m = (uint32_t *)(q + 1);   // dword boundary
+                m[0] = __builtin_amdgcn_alignbit(d1, d0, 16);
+                m[1] = __builtin_amdgcn_alignbit(d2, d1, 16);
+                m[2] = __builtin_amdgcn_alignbit(d3, d2, 16);
+                q[7] = (uint16_t)(d3 >> 16);
+            } else if (((uintptr_t)q & 2u) == 0) {   // row end at a 4-byte boundary: whole pairs, then the odd element
+                uint32_t *q32 = (uint32_t *)q;
+                if (nz > 1u) q32[0] = d0;
+                if (nz > 3u) q32[1] = d1;
+                if (nz > 5u) q32[2] = d2;
+                if (nz & 1u) {
+                    const uint32_t last = nz == 1u ? d0 : nz == 3u ? d1 : nz == 5u ? d2 : d3;
+                    q[nz - 1] = (uint16_t)(last & 0xffffu);
+                }
+            } else {   // row end at 2 mod 4: the first element, pairs (1,2) (3,4) as dwords, the last if unpaired
+                q[0] = (uint16_t)(d0 & 0xffffu);
+                uint32_t *m = (uint32_t *)(q + 1);   // dword boundary
+                if (nz > 2u) m[0] = __builtin_amdgcn_alignbit(d1, d0, 16);
+                if (nz > 4u) m[1] = __builtin_amdgcn_alignbit(d2, d1, 16);
+                if (nz > 6u) m[2] = __builtin_amdgcn_alignbit(d3, d2, 16);
+                if ((nz & 1u) == 0) {   // element nz - 1 (odd index) has no partner
+                    const uint32_t last = nz == 2u ? d0 : nz == 4u ? d1 : nz == 6u ? d2 : d3;
+                    q[nz - 1] = (uint16_t)(last >> 16);
+                }
             }
         }
     }

@@ -74,12 +74,14 @@ def test_long_lookback_chain(gpu, orc):
     check_case(gpu, orc, vol, (5, 5, 5, 5), 0.5)
 
 
-@pytest.mark.parametrize("shape", [(2, 64, 96), (2, 21, 30)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("shape", [(2, 64, 96), (2, 21, 30), (3, 40, 67), (2, 17, 1557)],
+                         ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("offset_elems", [1, 2, 8])
 def test_misaligned_input(gpu, orc, offset_elems, shape):
     """Volumes at 2-, 4- and 16-byte offsets inside a larger buffer use other load
-    paths; with z = 30 the row-end blocks are partial.  Decoding leaves the
-    bytes around the volume untouched."""
+    paths; with z = 30 the row-end blocks are partial; with z odd (67, 1557)
+    rows alternate between 4-byte and 2-byte alignment (shifted 16-byte loads,
+    split stores).  Decoding leaves the bytes around the volume untouched."""
     import torch
     vol = synth("cssi", shape, seed=11)
     big = torch.zeros(vol.size + 16, dtype=torch.int16, device="cuda")
