@@ -1146,9 +1146,9 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // header bytes alone.  A workgroup of 256 threads takes a chunk of 64 tiles
 // (4 threads per tile, TILE/4 header bytes each, read as 16-byte pieces so
 // that a wave's load covers 64 contiguous bytes of each of 16 tiles), reduces
-// sizes per tile, scans them, and chains chunks with the same look-back.  tile_off[t] is the
-// word offset of tile t's payload after the header section; tile_off[ntiles]
-// is the total.
+// sizes per tile, scans them, and chains chunks with the same look-back.
+// tile_off[t] is the word offset of tile t's payload after the header section;
+// tile_off[ntiles] is the total.
 // ---------------------------------------------------------------------------
 struct OffShared {
     uint32_t tagg[64];
