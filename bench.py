@@ -14,6 +14,7 @@ Prints ONE JSON line.  `--gpus` above 1 is refused: this build runs on one GPU.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -62,21 +63,28 @@ def profiled_traffic(dims):
     return prof.get("traffic_bytes")
 
 
-def cpu_baseline(vol, sample_frames, bins, pooling):
-    """Oracle (C, OpenMP two-pass) on the first `sample_frames` frames, all host threads."""
+def cpu_baseline(vol, sample_frames, bins, pooling, single_frames=8):
+    """Oracle (C, OpenMP two-pass) on the first `sample_frames` frames, all host
+    threads; and the serial oracle (one thread) on the first `single_frames`."""
     import numpy as np
     import oracle
     orc = oracle.get()
+
+    def rate(sub, omp, limit_s, max_reps):
+        orc.compress(sub, bins, pooling, omp=omp)           # warm-up
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            orc.compress(sub, bins, pooling, omp=omp)
+            reps += 1
+            if time.perf_counter() - t0 > limit_s or reps >= max_reps:
+                break
+        return (time.perf_counter() - t0) / reps
+
     sub = np.ascontiguousarray(vol[:sample_frames])
-    orc.compress(sub, bins, pooling, omp=True)           # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        st, _, n = orc.compress(sub, bins, pooling, omp=True)
-        reps += 1
-        if time.perf_counter() - t0 > 5.0 or reps >= 5:
-            break
-    dt = (time.perf_counter() - t0) / reps
-    return sub.nbytes / dt / 1e9, orc.max_threads(), sub.nbytes, dt
+    dt = rate(sub, True, 5.0, 5)
+    one = np.ascontiguousarray(vol[:min(single_frames, sample_frames)])
+    dt1 = rate(one, False, 3.0, 3)
+    return sub.nbytes / dt / 1e9, orc.max_threads(), sub.nbytes, dt, one.nbytes / dt1 / 1e9, one.shape[0]
 
 
 def bench_batched(torch, lscomp, datagen, np, steps):
@@ -225,7 +233,8 @@ def main():
 
     t = time.perf_counter()
     vol = datagen.volume("cssi", 2, dims)
-    log(f"generated {dims} in {time.perf_counter() - t:.1f}s")
+    vol_sha = hashlib.sha256(memoryview(vol).cast("B")).hexdigest()
+    log(f"generated {dims} in {time.perf_counter() - t:.1f}s, sha256 {vol_sha}")
     d_in = torch.from_numpy(vol.view(np.int16)).cuda()
     cap = lscomp.max_compressed_bytes(dims)
     d_out = torch.empty(cap, dtype=torch.uint8, device="cuda")
@@ -277,9 +286,15 @@ def main():
     # The library's plain copy makes 16-byte accesses per lane like the codec
     # kernels, so its FETCH_SIZE/WRITE_SIZE calibrate the counters for them.
     cp = torch.empty_like(d_in)
+    lscomp.copy_bytes(cp, d_in)   # first touch of the new buffer outside the timing
+    torch.cuda.synchronize()
     _, cp_ms, _ = timed_loop(torch, lambda: lscomp.copy_bytes(cp, d_in), 5)
     copy_gbps = 2 * in_bytes / (cp_ms * 1e-3) / 1e9
     del cp
+    # each mode's kernel rate also as a fraction of that measured copy rate
+    for r in results.values():
+        for d in ("compress", "decompress"):
+            r[f"{d}_copy_frac"] = r[f"{d}_hbm_frac"] * (HBM_PEAK / 1e9) / copy_gbps
 
     forms = {}
     if not args.no_forms:
@@ -295,10 +310,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         frames = min(args.cpu_frames, dims[0])
-        gbps, threads, nbytes, dt = cpu_baseline(vol, frames, (1, 1, 1, 1), 1.0)
+        gbps, threads, nbytes, dt, gbps1, frames1 = cpu_baseline(vol, frames, (1, 1, 1, 1), 1.0)
         cpu = {"value": gbps, "unit": "GB/s", "cores": threads, "kind": "port",
                "sample": f"lossless compression of the first {frames} frames ({nbytes / 1e9:.3f} GB), "
-                         f"OpenMP two-pass oracle, {dt:.3f} s per pass"}
+                         f"OpenMP two-pass oracle, {dt:.3f} s per pass",
+               "value_1_thread": gbps1,
+               "sample_1_thread": f"serial oracle, first {frames1} frames"}
     rec = {
         "metric": "compress/decompress GB/s + % of HBM peak on 1 MI355X; p99 per-frame latency",
         "value": ll["compress_GBps"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -306,7 +323,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
         "config": {"workload": f"CSSI-shaped {dims[0]}x{dims[1]}x{dims[2]} uint16, Poisson 0.3 + 50 peaks/frame "
                                f"+ gap stripes, seed 2; lossless compression (bins 1 1 1 1, pooling 1)",
-                   "dims": list(dims), "input_bytes": in_bytes},
+                   "dims": list(dims), "input_bytes": in_bytes, "volume_sha256": vol_sha},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(dims)},
         "cpu_baseline": cpu,
