@@ -8,18 +8,19 @@
 //   2. each lane loads its block once (8 16-byte row loads, coalesced across
 //      lanes: lanes are consecutive blocks along z), computes the header byte;
 //   3. workgroup scan of payload sizes, tile aggregate published at once;
-//   4. two tiles in flight per workgroup: the loads and packing of the next
-//      tile overlap the look-back of the current one, whose payload waits
-//      packed in LDS;
-//   5. decoupled look-back with a window of 512 predecessors per probe round
-//      (512/TILE per thread), one 64-bit word per tile (epoch | flag | value)
-//      read and written with relaxed agent-scope atomics, so no fences are
-//      needed on the multi-XCD part;
+//   4. a tile stays pending, packed in LDS, for two iterations while the
+//      workgroup loads and packs the next ones; its look-back runs then;
+//   5. decoupled look-back with a window of 128 predecessors per probe round,
+//      one 64-bit word per tile (epoch | flag | value) read and written with
+//      relaxed agent-scope atomics, so no fences are needed on the multi-XCD
+//      part;
 //   6. headers and payload leave LDS as contiguous coalesced 16-byte stores.
 // Decompression (replaces cuLSZ_decompression_kernel_uint16_bsize64,
-// cuLSZ_kernel.cu:1286-1646) runs the same scan over header sizes, stages the
-// tile's payload in LDS with coalesced loads, and each lane unpacks its block
-// and writes its rows.  Rate-0 blocks decode to zeros.
+// cuLSZ_kernel.cu:1286-1646) takes tile offsets from a pre-pass over the
+// header bytes, stages each tile's payload in LDS with coalesced loads (for
+// dense streams one tile ahead), and each lane unpacks its block and writes
+// its rows.  Rate-0 blocks decode to zeros.  uint32 volumes run the same
+// kernels with the uint32 codec.
 #include "lscomp_internal.h"
 
 namespace lsc {
