@@ -752,7 +752,8 @@ __device__ __forceinline__ void store_tile(const Item &it, uint32_t tl, uint32_t
 // per iteration parity; header bytes have four buffers (a tile's buffer is
 // rewritten four tiles later); a ring region is rewritten only by a later
 // iteration's pack, after its scan barrier.  (Timed alternatives: issuing the
-// next loads early, and storing p before packing n, were both slower.)
+// next loads early, storing p before packing n, and a closing barrier after
+// each iteration or after large tiles were all slower or no faster.)
 // Forward progress: a look-back waits only for lower tile ids; the lowest tile
 // without an inclusive prefix belongs to a workgroup that, whatever stage the
 // tile is in, either resolves it (all its predecessors are inclusive) or
@@ -763,12 +764,6 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
                                                const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
 {
     constexpr uint32_t kRing = (uint32_t)TILE * C::kMaxWords;   // staging words: one worst-case tile
-    // An iteration whose resolved tile stored at least this many payload words
-    // ends with a barrier.  Timed on the CSSI volume: without any closing
-    // barrier lossy compression (~0.1 KB per tile) ran 5 % faster and lossless
-    // (~8.6 KB per tile) 4 % slower; with it after large tiles only, both were
-    // at their best.
-    constexpr uint32_t kSyncWords = 512;
     __shared__ TileShared<TILE, C::kMaxWords> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid == 0) {
@@ -856,7 +851,6 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
                 if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
             }
             store_tile<TILE>(pit, p0.tl, p0.agg, sh.hdr[p0.hb], sh.pay + p0.start, excl, tid);
-            if (p0.agg >= kSyncWords) __syncthreads();
             p0 = p1;
             np--;
         }
