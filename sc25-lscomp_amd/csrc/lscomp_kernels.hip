@@ -440,14 +440,19 @@ __device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len
 }
 
 // Look-back window: one probe round reads the lb_win(TILE) predecessor words
-// nearest to the tile, one per thread of the first lb_win / 64 waves.  Tiles
-// are resolved two iterations after they publish their aggregates, so an
+// nearest to the tile, one per thread of the first lb_win / 64 waves (whole
+// waves: a wave's unread lanes would count as empty predecessors).  Tiles are
+// resolved two iterations after they publish their aggregates, so an
 // inclusive prefix is nearly always close by; every tile pays for the window's
 // status reads (agent-scope loads of lines all workgroups are writing).
-// Timed at tile 512 on the CSSI volume, compression lossless / lossy: 512
-// words 0.861 / 0.858 ms, 256 0.82 / 0.81, 128 0.80 / 0.79, 64 0.81 / 0.81.
+// Timed on the CSSI volume, compression lossless / lossy: at tile 512 (uint16)
+// 512 words 0.861 / 0.858 ms, 256 0.82 / 0.81, 128 0.78-0.79 / 0.81, 64
+// 0.79 / 0.81; at tile 256 (uint32) 256 words 1.68 / 1.65, 128 1.62 / 1.61,
+// 64 1.57-1.58 / 1.59, 32 1.59 / 1.62.  A quarter of the tile, at least 64.
 constexpr int lb_per(int) { return 1; }   // probe words per thread
-constexpr uint32_t lb_win(int tile) { return tile < 128 ? (uint32_t)tile : 128u; }
+constexpr uint32_t lb_win(int tile) { return tile >= 256 ? (uint32_t)tile / 4 : 64u; }
+static_assert(lb_win(128) % 64 == 0 && lb_win(256) % 64 == 0 && lb_win(512) % 64 == 0 && lb_win(128) <= 128,
+              "the look-back window covers whole waves of the tile");
 
 // LDS of the workgroup-wide look-back, per slice of 64 window positions (one
 // wave's k-th words): nearest inclusive and empty positions and the sum of the
