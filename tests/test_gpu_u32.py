@@ -63,6 +63,15 @@ def test_u32_shapes(gpu, orc, shape, kind, mode):
     check_u32(gpu, orc, synth_u32(kind, shape, seed), bins, p)
 
 
+def test_u32_cssi_frame_shape(gpu, orc):
+    """Sixteen 1813 x 1558 frames (the CSSI detector's frame: a partial block at
+    every row end) of photon counts scaled past 16 bits, lossless and lossy."""
+    vol = synth_u32("counts", (16, 1813, 1558), 11)
+    for bins, p in (((1, 1, 1, 1), 1.0), ((3, 5, 10, 15), 0.5)):
+        print(f"u32 frame-shape case bins={bins} p={p}", flush=True)
+        check_u32(gpu, orc, vol, bins, p)
+
+
 def test_u32_long_chain(gpu, orc):
     """128 and 512 tiles of 256 blocks: look-back and pre-pass over many tiles."""
     for shape in ((8, 512, 512), (32, 512, 512)):
