@@ -108,10 +108,27 @@ void lscomp_release_all(void);
  * autotuner or by hand. */
 int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu);
 int lscomp_get_launch_shape(int direction, uint32_t *tile, uint32_t *wg_per_cu);
-/* Single-volume decompression: 1 = tile offsets from a header pre-pass
- * (default), 0 = decoupled look-back inside the decode kernel.  Returns the
- * previous setting.  Output is identical either way. */
-int lscomp_set_decode_prepass(int enable);
+
+/* The full launch shape.  lb_window: look-back words read per probe round
+ * (a multiple of 64, at most tile; 0 = the default, 64).  counters: device
+ * counters that hand out tile ids (1..16; 0 = the default, 4 for compression
+ * and 2 for decompression): one same-address atomic per tile serializes near
+ * 80 tiles per microsecond, so tile ids are spread over several counters, each
+ * workgroup drawing from one.
+ * lscomp_set_launch_shape sets tile and wg_per_cu and the default window and
+ * counters.  Streams are identical for every shape.  No counterpart in the
+ * reference (cuLSZ_entry.cu:100-101, cuLSZ_kernel.h:7 fix its grid). */
+typedef struct lscomp_launch_shape {
+    uint32_t tile, wg_per_cu, lb_window, counters;
+} lscomp_launch_shape;
+int lscomp_set_launch_shape_ex(int direction, const lscomp_launch_shape *shape);
+int lscomp_get_launch_shape_ex(int direction, lscomp_launch_shape *shape);
+/* How single-volume decompression finds each tile's payload offset:
+ * 1 = from the header bytes in a first phase of the decode launch (default),
+ * 2 = from the header bytes in a separate pre-pass kernel, 0 = a decoupled
+ * look-back inside the decode kernel.  Returns the previous mode (negative
+ * for an invalid mode).  Output is identical in every mode. */
+int lscomp_set_decode_prepass(int mode);
 
 /* One measured launch shape (times in ms, averaged over the repetitions). */
 typedef struct lscomp_tune_entry {

@@ -46,15 +46,48 @@ uint64_t num_blocks(lscomp_dim3 d)
     return (uint64_t)d.x * (((uint64_t)d.y + 7) / 8) * (((uint64_t)d.z + 7) / 8);
 }
 
-// Launch shape per direction (0 compress, 1 decompress): tile and workgroups
-// per CU (0 = occupancy limit).  Set by lscomp_set_launch_shape / the autotuner.
-std::atomic<uint32_t> g_tile[2] = {{kDefaultTile[0]}, {kDefaultTile[1]}};
-std::atomic<uint32_t> g_wgpcu[2] = {{0u}, {0u}};
-// Single-volume decompression computes tile offsets in a pre-pass instead of
-// a look-back (LSCOMP_DECODE_PREPASS=0 in the environment turns it off).
-std::atomic<bool> g_decode_prepass{[] {
+// Launch shape per direction (0 compress, 1 decompress), packed 16 bits per
+// field (tile, workgroups per CU, window, counters) into one word.  Set by
+// lscomp_set_launch_shape[_ex] / the autotuner.
+namespace {
+uint64_t pack_shape(const Shape &s)
+{
+    return (uint64_t)(s.tile & 0xffffu) | (uint64_t)(s.wgpcu & 0xffffu) << 16 | (uint64_t)(s.win & 0xffffu) << 32 |
+           (uint64_t)(s.nc & 0xffffu) << 48;
+}
+Shape unpack_shape(uint64_t v)
+{
+    return Shape{(uint32_t)(v & 0xffffu), (uint32_t)(v >> 16 & 0xffffu), (uint32_t)(v >> 32 & 0xffffu),
+                 (uint32_t)(v >> 48 & 0xffffu)};
+}
+Shape default_shape(int d)
+{
+    return Shape{kDefaultTile[d], 0u, default_window(kDefaultTile[d]), kDefaultCounters[d]};
+}
+std::atomic<uint64_t> g_shape[2] = {{pack_shape(default_shape(0))}, {pack_shape(default_shape(1))}};
+}  // namespace
+
+Shape load_shape(int direction) { return unpack_shape(g_shape[direction].load()); }
+void store_shape(int direction, const Shape &s) { g_shape[direction].store(pack_shape(s)); }
+
+int normalize_shape(int direction, Shape &s)
+{
+    if (s.tile == 0) s.tile = kDefaultTile[direction];
+    if (s.tile != 128 && s.tile != 256 && s.tile != 512) return LSCOMP_ERR_INVALID_ARG;
+    if (s.wgpcu > 64) return LSCOMP_ERR_INVALID_ARG;
+    if (s.win == 0) s.win = default_window(s.tile);
+    if (s.win % 64 != 0 || s.win > s.tile) return LSCOMP_ERR_INVALID_ARG;
+    if (s.nc == 0) s.nc = kDefaultCounters[direction];
+    if (s.nc > kMaxCounters) return LSCOMP_ERR_INVALID_ARG;
+    return LSCOMP_OK;
+}
+// How a single-volume decompression finds its tile offsets: 1 = a first phase
+// of the decode launch (default), 2 = a separate pre-pass kernel, 0 = a
+// decoupled look-back inside the decode (LSCOMP_DECODE_PREPASS in the
+// environment sets the initial mode).
+std::atomic<int> g_decode_mode{[] {
     const char *e = std::getenv("LSCOMP_DECODE_PREPASS");
-    return !(e && e[0] == '0');
+    return (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
 }()};
 
 int hip_fail(hipError_t e, const char *what)
@@ -77,8 +110,9 @@ int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it, uint32_t t
     it.x = dims.x;
     it.y = dims.y;
     it.z = dims.z;
-    it.nbz = (dims.z + 7) / 8;
-    it.per_frame = ((dims.y + 7) / 8) * it.nbz;
+    // in 64 bits: y or z near 2^32 must not wrap (nb <= 2^31 - 1 bounds both results)
+    it.nbz = (uint32_t)(((uint64_t)dims.z + 7) / 8);
+    it.per_frame = (uint32_t)((((uint64_t)dims.y + 7) / 8) * it.nbz);
     it.nblocks = (uint32_t)nb;
     it.hdr_bytes = (uint32_t)((nb + 3) / 4 * 4);
     it.first_tile = 0;
@@ -90,10 +124,11 @@ int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it, uint32_t t
     return LSCOMP_OK;
 }
 
-LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, uint32_t tile, uint32_t wg_per_cu)
+LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, const Shape &shape)
 {
     static std::atomic<int> occ[2][2][3];   // [compress][table][tile 128/256/512]
     static std::atomic<int> ncu{0};
+    const uint32_t tile = shape.tile;
     const int ti = tile == 128 ? 0 : tile == 256 ? 1 : 2;
     int o = occ[compress][table][ti].load();
     if (o == 0) {
@@ -106,13 +141,17 @@ LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, uint32_t til
         ncu.store(n);
     }
     uint32_t per_cu = (uint32_t)o;
-    if (wg_per_cu && wg_per_cu < per_cu) per_cu = wg_per_cu;
+    if (shape.wgpcu && shape.wgpcu < per_cu) per_cu = shape.wgpcu;
     LaunchCfg c{};
     c.total_tiles = (uint32_t)total_tiles;
     c.tile = tile;
     uint64_t g = (uint64_t)(n > 0 ? n : 1) * per_cu;
     if (g > total_tiles) g = total_tiles;
     c.grid = (uint32_t)(g ? g : 1);
+    // every class of tile ids needs a workgroup: at most one counter per workgroup
+    c.sched.nc = shape.nc < c.grid ? shape.nc : c.grid;
+    if (c.sched.nc == 0) c.sched.nc = 1;
+    c.sched.win = shape.win ? shape.win : default_window(tile);
     return c;
 }
 
@@ -187,6 +226,7 @@ int ensure_tiles(Scratch *s, uint64_t tiles)
     if (!st) st = hip_fail(hipMalloc((void **)&s->tile_off, (cap + 1) * sizeof(uint64_t)), "hipMalloc");
     if (st) return st;
     st = hip_fail(launch_fill_u64(s->status, cap, 0, s->stream), "fill");
+    if (!st) st = hip_fail(launch_fill_u64(s->tile_off, cap + 1, 0, s->stream), "fill");
     if (st) return st;
     s->cap_tiles = cap;
     s->launches = 0;
@@ -198,6 +238,7 @@ int count_launch(Scratch *s)
 {
     if (++s->launches >= kEpochClearPeriod) {
         int st = hip_fail(launch_fill_u64(s->status, s->cap_tiles, 0, s->stream), "fill");
+        if (!st) st = hip_fail(launch_fill_u64(s->tile_off, s->cap_tiles + 1, 0, s->stream), "fill");
         if (st) return st;
         s->launches = 1;
     }
@@ -206,7 +247,7 @@ int count_launch(Scratch *s)
 
 int read_ctrl(Scratch *s)
 {
-    int st = hip_fail(hipMemcpyAsync(s->host, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream),
+    int st = hip_fail(hipMemcpyAsync(s->host, s->ctrl, kCtrlHead, hipMemcpyDeviceToHost, s->stream),
                       "hipMemcpyAsync");
     if (st) return st;
     return hip_fail(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
@@ -228,9 +269,9 @@ int take_errors(Scratch *s)
 
 // Validates a compression call and fills its Item (no HIP calls).
 int prepare_compress(const uint16_t *d_in, uint8_t *d_out, size_t cap, lscomp_dim3 dims, lscomp_bins4 bins,
-                     float p, Item &it)
+                     float p, uint32_t tile, Item &it)
 {
-    int st = make_item(dims, bins, p, it, g_tile[0].load());
+    int st = make_item(dims, bins, p, it, tile);
     if (st) return st;
     if (!d_in || !d_out || ((uintptr_t)d_in & 1) || ((uintptr_t)d_out & 3)) return LSCOMP_ERR_INVALID_ARG;
     if (cap < it.hdr_bytes) return LSCOMP_ERR_CAPACITY;
@@ -242,9 +283,9 @@ int prepare_compress(const uint16_t *d_in, uint8_t *d_out, size_t cap, lscomp_di
 }
 
 int prepare_decompress(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
-                       lscomp_bins4 bins, Item &it)
+                       lscomp_bins4 bins, uint32_t tile, Item &it)
 {
-    int st = make_item(dims, bins, 0.0f, it, g_tile[1].load());
+    int st = make_item(dims, bins, 0.0f, it, tile);
     if (st) return st;
     if (!d_in || !d_out || ((uintptr_t)d_out & 1) || ((uintptr_t)d_in & 3)) return LSCOMP_ERR_INVALID_ARG;
     if (in_bytes < it.hdr_bytes) return LSCOMP_ERR_CORRUPT;
@@ -303,35 +344,46 @@ int launch_item_u32(Scratch *s, Item &it, bool compress)
     uint64_t g = (uint64_t)(num_cus() > 0 ? num_cus() : 1) * (uint64_t)o;
     if (g > it.ntiles) g = it.ntiles;
     cfg.grid = (uint32_t)(g ? g : 1);
+    // the direction's counters; the window of 64 words timed best at this tile
+    const uint32_t nc = load_shape(compress ? 0 : 1).nc;
+    cfg.sched.nc = nc < cfg.grid ? (nc ? nc : 1u) : cfg.grid;
+    cfg.sched.win = default_window(kTileU32);
     if (compress) {
         if (!it.result) it.result = &s->ctrl->result;
     } else {
         cfg.tile_off = s->tile_off;
-        st = count_launch(s);   // the pre-pass is a launch of its own
-        if (st) return st;
+        cfg.off_mode = g_decode_mode.load() == 2 ? OFF_KERNEL : OFF_PHASE;
+        if (cfg.off_mode == OFF_KERNEL) {
+            st = count_launch(s);   // the pre-pass is a launch of its own
+            if (st) return st;
+        }
     }
     return hip_fail(launch_u32(compress, it, cfg, s->ctrl, s->status, s->stream),
                     compress ? "compress launch" : "decompress launch");
 }
 
-// Caller holds s->mu.  it.ntiles was computed for the direction's tile size.
-int launch_item(Scratch *s, Item &it, bool compress)
+// Caller holds s->mu.  it.ntiles was computed for shape.tile (the same shape
+// the call loaded once).
+int launch_item(Scratch *s, Item &it, bool compress, const Shape &shape)
 {
-    const int d = compress ? 0 : 1;
     int st = ensure_tiles(s, it.ntiles);
     if (st) return st;
     st = count_launch(s);
     if (st) return st;
-    const LaunchCfg cfg = make_cfg(it.ntiles, compress, false, g_tile[d].load(), g_wgpcu[d].load());
+    const LaunchCfg cfg = make_cfg(it.ntiles, compress, false, shape);
     if (compress) {
         if (!it.result) it.result = &s->ctrl->result;
         return hip_fail(launch_compress(it, nullptr, nullptr, cfg, s->ctrl, s->status, s->stream), "compress launch");
     }
     LaunchCfg dcfg = cfg;
-    if (g_decode_prepass.load()) {
+    const int mode = g_decode_mode.load();
+    if (mode != 0) {
         dcfg.tile_off = s->tile_off;
-        st = count_launch(s);   // the pre-pass is a launch of its own
-        if (st) return st;
+        dcfg.off_mode = mode == 2 ? OFF_KERNEL : OFF_PHASE;
+        if (mode == 2) {
+            st = count_launch(s);   // the pre-pass is a launch of its own
+            if (st) return st;
+        }
     }
     return hip_fail(launch_decompress(it, nullptr, nullptr, dcfg, s->ctrl, s->status, s->stream), "decompress launch");
 }
@@ -354,28 +406,30 @@ int lscomp_compress_u16_async(const uint16_t *d_in, uint8_t *d_out, size_t out_c
                               lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream)
 {
     if (!d_out_bytes || ((uintptr_t)d_out_bytes & 7)) return LSCOMP_ERR_INVALID_ARG;
+    const Shape shape = load_shape(0);
     Item it;
-    int st = prepare_compress(d_in, d_out, out_capacity, dims, bins, pooling_th, it);
+    int st = prepare_compress(d_in, d_out, out_capacity, dims, bins, pooling_th, shape.tile, it);
     if (st) return st;
     it.result = d_out_bytes;
     Scratch *s = get_scratch(stream);
     if (!s) return LSCOMP_ERR_HIP;
     std::lock_guard<std::mutex> lk(s->mu);
-    return launch_item(s, it, true);
+    return launch_item(s, it, true, shape);
 }
 
 int lscomp_compress_u16(const uint16_t *d_in, uint8_t *d_out, size_t out_capacity, size_t *out_bytes,
                         lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, hipStream_t stream)
 {
     if (!out_bytes) return LSCOMP_ERR_INVALID_ARG;
+    const Shape shape = load_shape(0);
     Item it;
-    int st = prepare_compress(d_in, d_out, out_capacity, dims, bins, pooling_th, it);
+    int st = prepare_compress(d_in, d_out, out_capacity, dims, bins, pooling_th, shape.tile, it);
     if (st == LSCOMP_ERR_CAPACITY) *out_bytes = lscomp_header_bytes_u16(dims);
     if (st) return st;
     Scratch *s = get_scratch(stream);
     if (!s) return LSCOMP_ERR_HIP;
     std::lock_guard<std::mutex> lk(s->mu);
-    st = launch_item(s, it, true);
+    st = launch_item(s, it, true, shape);
     if (st) return st;
     st = read_ctrl(s);
     if (st) return st;
@@ -388,25 +442,27 @@ int lscomp_compress_u16(const uint16_t *d_in, uint8_t *d_out, size_t out_capacit
 int lscomp_decompress_u16_async(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
                                 lscomp_bins4 bins, hipStream_t stream)
 {
+    const Shape shape = load_shape(1);
     Item it;
-    int st = prepare_decompress(d_out, d_in, in_bytes, dims, bins, it);
+    int st = prepare_decompress(d_out, d_in, in_bytes, dims, bins, shape.tile, it);
     if (st) return st;
     Scratch *s = get_scratch(stream);
     if (!s) return LSCOMP_ERR_HIP;
     std::lock_guard<std::mutex> lk(s->mu);
-    return launch_item(s, it, false);
+    return launch_item(s, it, false, shape);
 }
 
 int lscomp_decompress_u16(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
                           lscomp_bins4 bins, hipStream_t stream)
 {
+    const Shape shape = load_shape(1);
     Item it;
-    int st = prepare_decompress(d_out, d_in, in_bytes, dims, bins, it);
+    int st = prepare_decompress(d_out, d_in, in_bytes, dims, bins, shape.tile, it);
     if (st) return st;
     Scratch *s = get_scratch(stream);
     if (!s) return LSCOMP_ERR_HIP;
     std::lock_guard<std::mutex> lk(s->mu);
-    st = launch_item(s, it, false);
+    st = launch_item(s, it, false, shape);
     if (st) return st;
     st = read_ctrl(s);
     if (st) return st;
@@ -524,24 +580,46 @@ void lscomp_release_all(void)
 int lscomp_set_launch_shape(int direction, uint32_t tile, uint32_t wg_per_cu)
 {
     if (direction != 0 && direction != 1) return LSCOMP_ERR_INVALID_ARG;
-    if (tile == 0) tile = kDefaultTile[direction];
-    if (tile != 128 && tile != 256 && tile != 512) return LSCOMP_ERR_INVALID_ARG;
-    g_tile[direction].store(tile);
-    g_wgpcu[direction].store(wg_per_cu);
+    Shape s{tile, wg_per_cu, 0u, 0u};
+    const int st = normalize_shape(direction, s);
+    if (st) return st;
+    store_shape(direction, s);
     return LSCOMP_OK;
 }
 
-int lscomp_set_decode_prepass(int enable)
+int lscomp_set_launch_shape_ex(int direction, const lscomp_launch_shape *shape)
 {
-    const bool old = g_decode_prepass.exchange(enable != 0);
-    return old ? 1 : 0;
+    if ((direction != 0 && direction != 1) || !shape) return LSCOMP_ERR_INVALID_ARG;
+    Shape s{shape->tile, shape->wg_per_cu, shape->lb_window, shape->counters};
+    const int st = normalize_shape(direction, s);
+    if (st) return st;
+    store_shape(direction, s);
+    return LSCOMP_OK;
+}
+
+int lscomp_get_launch_shape_ex(int direction, lscomp_launch_shape *shape)
+{
+    if ((direction != 0 && direction != 1) || !shape) return LSCOMP_ERR_INVALID_ARG;
+    const Shape s = load_shape(direction);
+    shape->tile = s.tile;
+    shape->wg_per_cu = s.wgpcu;
+    shape->lb_window = s.win;
+    shape->counters = s.nc;
+    return LSCOMP_OK;
+}
+
+int lscomp_set_decode_prepass(int mode)
+{
+    if (mode < 0 || mode > 2) return -LSCOMP_ERR_INVALID_ARG;
+    return g_decode_mode.exchange(mode);
 }
 
 int lscomp_get_launch_shape(int direction, uint32_t *tile, uint32_t *wg_per_cu)
 {
     if ((direction != 0 && direction != 1) || !tile || !wg_per_cu) return LSCOMP_ERR_INVALID_ARG;
-    *tile = g_tile[direction].load();
-    *wg_per_cu = g_wgpcu[direction].load();
+    const Shape s = load_shape(direction);
+    *tile = s.tile;
+    *wg_per_cu = s.wgpcu;
     return LSCOMP_OK;
 }
 

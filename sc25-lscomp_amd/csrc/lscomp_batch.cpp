@@ -18,7 +18,7 @@ using namespace lsc;
 struct lscomp_plan_s {
     int device = 0;
     int direction = 0;            // 0 compress, 1 decompress
-    uint32_t tile = kDefaultTile[0], wgpcu = 0;
+    Shape shape{};                // launch shape, fixed at creation (a captured graph keeps it)
     uint32_t max_items = 0;
     uint64_t max_tiles = 0;
     uint32_t nitems = 0, ntiles = 0;
@@ -82,7 +82,7 @@ LaunchCfg plan_cfg(lscomp_plan p)
 {
     // Persistent grid sized for the plan's capacity so that a captured graph
     // stays valid for every later batch; the tile count comes from d_total.
-    LaunchCfg c = make_cfg(p->max_tiles, p->direction == 0, true, p->tile, p->wgpcu);
+    LaunchCfg c = make_cfg(p->max_tiles, p->direction == 0, true, p->shape);
     c.total_tiles = 0;
     c.d_total_tiles = p->d_total;
     return c;
@@ -107,11 +107,10 @@ int lscomp_plan_create(int direction, uint32_t max_items, uint64_t max_blocks, l
         return LSCOMP_ERR_INVALID_ARG;
     lscomp_plan p = new lscomp_plan_s;
     p->direction = direction;
-    p->tile = g_tile[direction].load();
-    p->wgpcu = g_wgpcu[direction].load();
+    p->shape = load_shape(direction);
     p->max_items = max_items;
     // every item wastes at most one partial tile
-    p->max_tiles = (max_blocks + p->tile - 1) / p->tile + max_items;
+    p->max_tiles = (max_blocks + p->shape.tile - 1) / p->shape.tile + max_items;
     if (p->max_tiles > 0xffffffffull) { delete p; return LSCOMP_ERR_TOO_LARGE; }
     if (hip_fail(hipGetDevice(&p->device), "hipGetDevice")) { delete p; return LSCOMP_ERR_HIP; }
     p->stage_bytes = table_bytes(max_items, p->max_tiles);
@@ -152,7 +151,7 @@ int lscomp_plan_set_items_u16(lscomp_plan p, const lscomp_item_u16 *items, uint3
         const lscomp_item_u16 &src = items[i];
         Item &it = h_items[i];
         const bool comp = p->direction == 0;
-        st = make_item(src.dims, src.bins, comp ? src.pooling_th : 0.0f, it, p->tile);
+        st = make_item(src.dims, src.bins, comp ? src.pooling_th : 0.0f, it, p->shape.tile);
         if (st) return st;
         if (!src.d_volume || !src.d_stream || ((uintptr_t)src.d_volume & 1) || ((uintptr_t)src.d_stream & 3))
             return LSCOMP_ERR_INVALID_ARG;
@@ -244,7 +243,7 @@ int lscomp_plan_status(lscomp_plan p, hipStream_t stream)
 {
     if (!p) return LSCOMP_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(p->mu);
-    int st = hip_fail(hipMemcpyAsync(p->h_ctrl, p->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream), "readback");
+    int st = hip_fail(hipMemcpyAsync(p->h_ctrl, p->ctrl, kCtrlHead, hipMemcpyDeviceToHost, stream), "readback");
     if (st) return st;
     st = hip_fail(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (st) return st;

@@ -70,6 +70,38 @@ LSC_HD uint32_t bit_width(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz
 // ---------------------------------------------------------------------------
 LSC_HD uint32_t quant(uint32_t v, float f) { return (uint32_t)((float)v * f); }
 
+// ---------------------------------------------------------------------------
+// Division by a per-block bin with a 24-bit multiply-high (v_mul_hi_u32_u24),
+// used when every bin lies in [2, 16384]:
+//   floor(s / d) = mulhi24(s * 2^a, m),  m = ceil(2^(32-a) / d),
+//   mulhi24(x, y) = floor(x[23:0] * y[23:0] / 2^32).
+// Proof: s*2^a*m / 2^32 = s/d + s*e / (d*2^(32-a)) with e = m*d - 2^(32-a) in
+// [0, d); writing s = q*d + r, the value is q + (r + s*e/2^(32-a)) / d, and
+// s*e < 2^(32-a) keeps r + s*e/2^(32-a) < r + 1 <= d, so its floor is q.
+// For s < 2^S and d in (2^(j-1), 2^j], any a with 9-j <= a <= min(24-S,
+// 32-S-j) gives s*2^a < 2^24, m < 2^24 and s*e < 2^(32-a); make_div24 picks
+// the largest and checks the three bounds.  Plain values (S = 16) cover d in
+// [2, 65536], pair sums (S = 17) d in [3, 32768], so bins in [2, 16384] cover
+// every plain bin and every pooled divisor 2*bin.  Checked exhaustively on
+// the CPU for every such divisor and every s.  The scaling s*2^a
+// of the two halves of a packed pair is one v_dot2_u32_u16 with {2^a, 2^a}
+// (the pair sum: pooled) or {2^a, 0} / {0, 2^a} (one element: plain).
+// ---------------------------------------------------------------------------
+LSC_HD uint32_t mulhi24(uint32_t a, uint32_t b)
+{
+    return (uint32_t)(((uint64_t)(a & 0xffffffu) * (uint64_t)(b & 0xffffffu)) >> 32);
+}
+
+// a.x * b.x + a.y * b.y + c over the 16-bit halves (v_dot2_u32_u16)
+LSC_HD uint32_t dot2(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_udot2(as_u16x2(a), as_u16x2(b), c, false);
+#else
+    return (a & 0xffffu) * (b & 0xffffu) + (a >> 16) * (b >> 16) + c;
+#endif
+}
+
 // x[k] for a per-block k in 0..3, as a select between values.  The empty asm
 // keeps them values: the compiler would otherwise load through a selected
 // address into the kernel's parameter block, which then has to live in
@@ -88,6 +120,25 @@ LSC_HD float pick4(const float x[4], uint32_t k)
     uint32_t u[4];
     for (int i = 0; i < 4; i++) u[i] = __builtin_bit_cast(uint32_t, x[i]);
     return __builtin_bit_cast(float, pick4(u, k));
+}
+
+// x[k] of a per-block k: pick4 for parameters passed in the kernel arguments
+// (kMem = false), a plain indexed load for parameters that live in device
+// memory (kMem = true: the batched form's item table), which needs no
+// registers for the four candidates.  The kMem kernels also keep the float
+// division: with the multiply-high path the batched compression kernel needs
+// more than its 128 registers and spills.
+template <bool kMem>
+LSC_HD uint32_t sel4(const uint32_t x[4], uint32_t k)
+{
+    if constexpr (kMem) return x[k];
+    else return pick4(x, k);
+}
+template <bool kMem>
+LSC_HD float sel4(const float x[4], uint32_t k)
+{
+    if constexpr (kMem) return x[k];
+    else return pick4(x, k);
 }
 
 // Fast u32 division by an invariant divisor (round-up multiplier method,
@@ -123,14 +174,22 @@ struct Decide {
     uint32_t zth;      // pool iff zero_count >= zth (65: never)
     uint32_t flags;    // DEC_*
     FastDiv fd[4];     // exact 32-bit division by b[k] (uint32 volumes)
+    // DEC_Q24: floor(v / b[k]) = mulhi24(dot2(pair, qs[k] or qs[k] << 16), qm[k])
+    // for one element of a packed pair, floor((a + b) / 2b[k]) =
+    // mulhi24(dot2(pair, ps[k]), pm[k]) for a pooled pair (see mulhi24)
+    uint32_t qm[4], qs[4];   // plain: multiplier, 2^a (low half of the scale word)
+    uint32_t pm[4], ps[4];   // pooled: multiplier, {2^a, 2^a}
 };
 enum : uint32_t {
     DEC_LOSSLESS = 1u,  // all bins 1 and no pooling: q == v
     DEC_POOL = 2u,      // pooling possible (zth <= 64)
     DEC_BINSEL = 4u,    // bin choice depends on the data (b0 != b3, b0 < 2^16)
+    DEC_Q24 = 8u,       // every bin in [2, 16384]: 24-bit multiply-high division
 };
 
-// Header byte of a block: (pool << 7) | (bin_choice << 5) | rate.
+// Header byte of a block: (pool << 7) | (bin_choice << 5) | rate.  kMem: p
+// lives in device memory (see sel4).
+template <bool kMem = false>
 LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
 {
     // max of all 64 values (packed 16-bit max)
@@ -142,8 +201,10 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
 
     uint32_t pool = 0, s2 = 0;
     if (p.flags & DEC_POOL) {
-        // nonzero count (min(v,1) summed per half) and max pair sum
-        u16x2 nz = {0, 0};
+        // nonzero count: min(v,1) per half, two packed pairs summed per 32-bit
+        // three-input add (no carry between the halves: each stays <= 32);
+        // max pair sum
+        uint32_t nzw = 0;
         uint32_t one_w = 0x00010001u;
 #if defined(__HIP_DEVICE_COMPILE__)
         // an opaque 1: with a visible constant the packed min becomes a
@@ -152,32 +213,45 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
 #endif
         const u16x2 one = as_u16x2(one_w);
 #pragma unroll
-        for (int i = 0; i < 32; i++) {
-            u16x2 v = as_u16x2(D[i]);
-            nz += __builtin_elementwise_min(v, one);
-            uint32_t s = (uint32_t)v.x + (uint32_t)v.y;
-            s2 = s > s2 ? s : s2;
+        for (int i = 0; i < 32; i += 2) {
+            const u16x2 v = as_u16x2(D[i]), w = as_u16x2(D[i + 1]);
+            nzw += as_u32(__builtin_elementwise_min(v, one)) + as_u32(__builtin_elementwise_min(w, one));
+            const uint32_t s = (uint32_t)v.x + (uint32_t)v.y, t = (uint32_t)w.x + (uint32_t)w.y;
+            const uint32_t st = s > t ? s : t;
+            s2 = st > s2 ? st : s2;
         }
-        uint32_t zc = 64u - ((uint32_t)nz.x + (uint32_t)nz.y);
+        const uint32_t zc = 64u - ((nzw & 0xffffu) + (nzw >> 16));
         pool = zc >= p.zth;
     }
 
     uint32_t bc = 3;
     if (p.flags & DEC_BINSEL) {
         // bin choice: bc = #{k in 1..3 : m0 >= b[k]}, m0 = min{v : v >= b0}
-        // (no value in [b0,b1) <=> m0 >= b1; bins are sorted).  Computed as
-        // min over (v - b0) mod 2^16, which is < 2^16 - b0 iff some v >= b0.
-        const u16x2 b0 = {(unsigned short)p.b[0], (unsigned short)p.b[0]};
-        u16x2 mt = as_u16x2(D[0]) - b0;
+        // (no value in [b0,b1) <=> m0 >= b1; bins are sorted).  When m1 < b1
+        // the maximum decides it: no value >= b0 (bc = 3), or m0 <= m1 < b1
+        // (bc = 0); only blocks with a value >= b1 need m0 (on the GPU a wave
+        // none of whose blocks has one skips the loop).  m0 is the minimum of
+        // (v - b0) mod 2^16, which is < 2^16 - b0 iff some v >= b0, plus b0.
+        bc = m1 < p.b[0] ? 3u : 0u;
+        if (m1 >= p.b[1]) {
+            const u16x2 b0 = {(unsigned short)p.b[0], (unsigned short)p.b[0]};
+            u16x2 mt = as_u16x2(D[0]) - b0;
 #pragma unroll
-        for (int i = 1; i < 32; i++) mt = __builtin_elementwise_min(mt, as_u16x2(D[i]) - b0);
-        uint32_t t = mt.x < mt.y ? mt.x : mt.y;
-        uint32_t m0 = (t <= 65535u - p.b[0]) ? t + p.b[0] : 0xffffffffu;
-        bc = (uint32_t)(m0 >= p.b[1]) + (uint32_t)(m0 >= p.b[2]) + (uint32_t)(m0 >= p.b[3]);
+            for (int i = 1; i < 32; i++) mt = __builtin_elementwise_min(mt, as_u16x2(D[i]) - b0);
+            const uint32_t t = mt.x < mt.y ? mt.x : mt.y;
+            const uint32_t m0 = (t <= 65535u - p.b[0]) ? t + p.b[0] : 0xffffffffu;
+            bc = (uint32_t)(m0 >= p.b[1]) + (uint32_t)(m0 >= p.b[2]) + (uint32_t)(m0 >= p.b[3]);
+        }
     }
     // max floor(floor((a+b)/2) / bin) = floor(max(a+b) / 2bin)
-    const float f = pool ? pick4(p.f2, bc) : pick4(p.f, bc);
-    uint32_t maxq = quant(pool ? s2 : m1, f);
+    uint32_t maxq;
+    if (!kMem && (p.flags & DEC_Q24)) {
+        const uint32_t sc = pool ? sel4<kMem>(p.ps, bc) & 0xffffu : sel4<kMem>(p.qs, bc);
+        maxq = mulhi24((pool ? s2 : m1) * sc, pool ? sel4<kMem>(p.pm, bc) : sel4<kMem>(p.qm, bc));
+    } else {
+        const float f = pool ? sel4<kMem>(p.f2, bc) : sel4<kMem>(p.f, bc);
+        maxq = quant(pool ? s2 : m1, f);
+    }
     return (pool << 7) | (bc << 5) | bit_width(maxq);
 }
 
@@ -272,12 +346,30 @@ LSC_HD void quantize_plain(uint32_t D[32], float f)
     }
 }
 
+// The same with the 24-bit multiply-high: scale word sc = 2^a of the bin
+// (low half), multiplier m.  Five instructions per pair of values.
+LSC_HD void quantize_plain24(uint32_t D[32], uint32_t sc, uint32_t m)
+{
+    const uint32_t sc_hi = sc << 16;
+#pragma unroll
+    for (int i = 0; i < 32; i++)
+        D[i] = perm(mulhi24(dot2(D[i], sc_hi, 0u), m), mulhi24(dot2(D[i], sc, 0u), m), 0x05040100u);
+}
+
 // Pooled: Q[t] = floor((v[2t] + v[2t+1]) / 2) / bin = floor((v[2t] + v[2t+1]) / 2bin),
 // t = 4r + c (pair c of row r); f2 = the pooled factor of the bin.
 LSC_HD void quantize_pooled(const uint32_t D[32], float f2, uint32_t Q[32])
 {
 #pragma unroll
     for (int t = 0; t < 32; t++) Q[t] = quant((D[t] & 0xffffu) + (D[t] >> 16), f2);
+}
+
+// The same with the 24-bit multiply-high: the pair sum scaled by 2^a in one
+// dot2 (scale word {2^a, 2^a}), then the multiply-high by m.
+LSC_HD void quantize_pooled24(const uint32_t D[32], uint32_t sc, uint32_t m, uint32_t Q[32])
+{
+#pragma unroll
+    for (int t = 0; t < 32; t++) Q[t] = mulhi24(dot2(D[t], sc, 0u), m);
 }
 
 // Pooled column words: C[u] byte m = low (hi = false) or high byte of Q[8m+u].
@@ -295,17 +387,18 @@ LSC_HD void pooled_columns(const uint32_t Q[32], uint32_t C[8], bool hi)
 
 // Encode the payload of one block into words out[0..]; returns the number of
 // 32-bit words (2*rate plain, rate pooled).  hdr from block_header().
-// Quantizes D in place for plain blocks.
+// Quantizes D in place for plain blocks.  kMem: p lives in device memory.
+template <bool kMem = false>
 LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint32_t *out)
 {
     const uint32_t rate = hdr & 0x1fu;
     if (rate == 0) return 0;
     const uint32_t bc = (hdr >> 5) & 3u;
-    const bool identity = pick4(p.b, bc) == 1u;
-    const float f = pick4(p.f, bc);
+    const bool identity = sel4<kMem>(p.b, bc) == 1u;
     if (hdr & 0x80u) {
         uint32_t Q[32], C[8];
-        quantize_pooled(D, pick4(p.f2, bc), Q);
+        if (!kMem && (p.flags & DEC_Q24)) quantize_pooled24(D, sel4<kMem>(p.ps, bc), sel4<kMem>(p.pm, bc), Q);
+        else quantize_pooled(D, sel4<kMem>(p.f2, bc), Q);
         pooled_columns(Q, C, false);
         const uint32_t nlo = rate < 8 ? rate : 8u;
         for (uint32_t i = 0; i < nlo; i++) out[i] = plane_from_columns(C, i);
@@ -315,7 +408,8 @@ LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint
         }
         return rate;
     }
-    if (!identity) quantize_plain(D, f);
+    if (!kMem && (p.flags & DEC_Q24)) quantize_plain24(D, sel4<kMem>(p.qs, bc), sel4<kMem>(p.qm, bc));   // no bin is 1
+    else if (!identity) quantize_plain(D, sel4<kMem>(p.f, bc));
     uint32_t C[16];
     columns_from_rows(D, C, false);
     const uint32_t nlo = rate < 8 ? rate : 8u;

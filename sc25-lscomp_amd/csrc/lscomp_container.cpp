@@ -56,6 +56,7 @@ bool item_ok(const lscomp_container_item &it)
     if (!b.x || !b.y || !b.z || !b.w || !(b.x <= b.y && b.y <= b.z && b.z <= b.w)) return false;
     if (!(it.pooling_th >= 0.0f && it.pooling_th <= 1.0f)) return false;
     const uint64_t nb = blocks_of(it.dims);
+    if (nb > 0x7fffffffull) return false;   // the codec's limit per volume
     const uint64_t R = (nb + 3) / 4 * 4;
     const uint64_t worst = R + (it.elem_bytes == 2 ? 128 : 248) * nb;
     return it.stream_bytes >= R && it.stream_bytes <= worst;

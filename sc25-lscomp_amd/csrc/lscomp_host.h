@@ -9,8 +9,14 @@
 
 namespace lsc {
 
-extern std::atomic<uint32_t> g_tile[2];    // blocks per tile, per direction
-extern std::atomic<uint32_t> g_wgpcu[2];   // workgroups per CU (0: occupancy limit)
+// Launch shape of a direction (0 compress, 1 decompress).  It is kept packed
+// in one atomic word, so a call reads a consistent shape with a single load
+// and uses that one copy for its tile count, scratch size and launch.
+Shape load_shape(int direction);
+void store_shape(int direction, const Shape &s);
+// Validates a shape and fills the direction's defaults (tile 0 -> 512,
+// window 0 -> default_window, counters 0 -> kDefaultCounters); returns a status.
+int normalize_shape(int direction, Shape &s);
 
 int hip_fail(hipError_t e, const char *what);
 bool bins_ok(lscomp_bins4 b);
@@ -18,6 +24,7 @@ uint32_t align_class(const void *vol, uint32_t z);
 uint64_t num_blocks(lscomp_dim3 d);
 // Validates dims/bins/pooling and fills the geometry and decision parts of an Item.
 int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it, uint32_t tile);
-LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, uint32_t tile, uint32_t wg_per_cu);
+// Grid and scheduling of a launch of `total_tiles` tiles with a normalized shape.
+LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, const Shape &shape);
 
 }  // namespace lsc

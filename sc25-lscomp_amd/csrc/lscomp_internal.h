@@ -16,24 +16,47 @@ constexpr uint32_t kMaxBlocksPerItem = 0x7fffffffu;   // prefix values fit 36-bi
 // Alignment class of a volume: how a row of one block (8 x u16) may be loaded.
 enum : uint32_t { ALIGN2 = 0, ALIGN4 = 1, ALIGN16 = 2 };
 
+// Tile ids of a launch come from `nc` counters: tile k*nc + c is the k-th id
+// of counter c, and a workgroup draws from counter c = (its ticket, i.e. its
+// place in the start order) mod nc.  One same-address device-scope atomic
+// serializes at about 12.5 ns, which caps a single counter near 80 tiles per
+// microsecond; counters on separate 256-byte lines spread that load.  Classes
+// follow the start order, so the first nc running workgroups cover every
+// class and a tile id is only ever handed to a running workgroup.
+constexpr uint32_t kMaxCounters = 16;
+struct CtrLine {
+    uint32_t v;
+    uint32_t pad[63];
+};
+
 // Launch-state block of a scratch area.  All fields start at zero; the last
 // workgroup of every launch resets the counters and advances the epoch, so a
 // launch needs no host-side clearing (and replays inside a hipGraph).
-// Each field that many workgroups touch sits on its own 128-byte line: the
-// tile counter takes one atomic per tile and must not share its line with
-// anything else that is read or written concurrently.
+// Each field that many workgroups touch sits on its own line: a tile counter
+// takes one atomic per tile and must not share its line with anything else
+// that is read or written concurrently.  The host reads back only the first
+// 16 bytes (error, result).
 struct Ctrl {
-    uint32_t tile_ctr;   // dynamic tile ids
-    uint32_t pad0[31];
-    uint32_t done_ctr;   // workgroups finished
-    uint32_t epoch;      // tags look-back words of the current launch (read once per workgroup)
-    uint32_t pad1[30];
     uint32_t error;      // sticky ERRF_* bits, cleared by the host
     uint32_t pad2;
     uint64_t result;     // stream bytes of a single-item compression
     uint64_t pad3[14];
+    uint32_t done_ctr;   // workgroups finished
+    uint32_t epoch;      // tags look-back words of the current launch (read once per workgroup)
+    uint32_t pad1[30];
+    uint32_t ticket;     // class of each workgroup (start order)
+    uint32_t pad0[63];
+    CtrLine chunk;       // decode: chunks of the tile-offset phase
+    CtrLine ctr[kMaxCounters];   // tile ids, one counter per class
 };
-static_assert(sizeof(Ctrl) == 384, "Ctrl is three 128-byte lines");
+static_assert(sizeof(Ctrl) == 768 + 256 * kMaxCounters, "Ctrl layout");
+constexpr size_t kCtrlHead = 16;   // bytes of Ctrl the host reads back (error, result)
+
+// Scheduling parameters of a launch (uniform over the grid).
+struct Sched {
+    uint32_t nc;    // tile-id counters, 1 .. kMaxCounters
+    uint32_t win;   // look-back window: predecessor words per probe round, a multiple of 64, <= tile
+};
 // ERRF_RANGE: a uint32 block's quantized maximum needs 32 bits (rate 32).
 enum : uint32_t { ERRF_CORRUPT = 1u, ERRF_TIMEOUT = 2u, ERRF_RANGE = 4u };
 
@@ -63,14 +86,36 @@ struct Item {
     Decide dec;
 };
 
+// Launch shape of one direction: the stream never depends on it.
+struct Shape {
+    uint32_t tile;      // blocks per tile (= threads per workgroup): 128, 256 or 512
+    uint32_t wgpcu;     // workgroups per CU (0: occupancy limit)
+    uint32_t win;       // look-back window (0: default for the tile)
+    uint32_t nc;        // tile-id counters (0: default)
+};
+// Default tile-id counters per direction (compression, decompression) and
+// look-back window.  Timed on the 600x1813x1558 CSSI volume (tile 512, one
+// MI355X, the same box for every variant): compression lossless 0.807 /
+// 0.781 / 0.780 ms with 1 / 2 / 4 counters (lossy 0.829 / 0.825 / 0.827),
+// decompression lossless 0.919 / 0.898 / 0.900 ms (lossy 0.750 / 0.742 /
+// 0.744); a window of 64 words against 128: lossless compression 0.780
+// against 0.790 ms, lossy 0.827 against 0.823 ms.
+constexpr uint32_t kDefaultCounters[2] = {4, 2};
+constexpr uint32_t default_window(uint32_t) { return 64u; }
+
 struct LaunchCfg {
     uint32_t grid;            // workgroups (persistent: each loops over tiles)
     uint32_t total_tiles;
     uint32_t tile;            // blocks per tile: 128, 256 or 512
+    Sched sched;              // counters and look-back window
     const uint32_t *d_total_tiles;   // if set, the tile count is read from device memory
-    uint64_t *tile_off;              // decompress, single item: if set, a pre-pass computes
-                                     // tile offsets (ntiles + 1 words) and the decode needs no look-back
+    uint64_t *tile_off;              // decompress, single item: if set, tile offsets (ntiles + 1
+                                     // epoch-tagged words) are computed first and the decode needs no look-back
+    uint32_t off_mode;               // with tile_off: OFF_PHASE or OFF_KERNEL
 };
+// How a single-volume decode gets its tile offsets: a first phase of the decode
+// launch (OFF_PHASE) or a separate pre-pass kernel (OFF_KERNEL).
+enum : uint32_t { OFF_PHASE = 1, OFF_KERNEL = 2 };
 
 // Kernel launchers (lscomp_kernels.hip).  `items` is a device array of item
 // descriptors and `tile_items` (device, total_tiles entries) the item of each

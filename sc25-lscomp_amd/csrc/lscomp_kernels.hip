@@ -294,8 +294,10 @@ __device__ __forceinline__ void store_block_u32(const Item &it, uint32_t bx, uin
 
 // What the tile kernels need of a value type: registers per block, payload
 // words per block at most, and the per-block load, header, encode, decode and
-// store.
-struct CodecU16 {
+// store.  kMem: the item descriptors live in device memory (table mode), so
+// per-block parameters are read with indexed loads (sel4).
+template <bool kMem>
+struct CodecU16T {
     static constexpr int kRegs = 32;              // 64 values as packed pairs
     static constexpr uint32_t kMaxWords = 32;     // 16 planes x 8 bytes
     __device__ __forceinline__ static void load(const Item &it, uint32_t bx, uint32_t by, uint32_t bz, uint32_t R[kRegs])
@@ -304,11 +306,11 @@ struct CodecU16 {
     }
     __device__ __forceinline__ static uint32_t header(const uint32_t R[kRegs], const Item &it, Ctrl *)
     {
-        return block_header(R, it.dec);
+        return block_header<kMem>(R, it.dec);
     }
     __device__ __forceinline__ static void encode(uint32_t R[kRegs], uint32_t hdr, const Item &it, uint32_t *out)
     {
-        encode_block(R, hdr, it.dec, out);
+        encode_block<kMem>(R, hdr, it.dec, out);
     }
     __device__ __forceinline__ static bool bad_header(uint32_t hdr) { return (hdr & 0x1fu) > 16u; }
     __device__ __forceinline__ static void decode(const uint32_t *in, uint32_t hdr, const Item &it, uint32_t R[kRegs])
@@ -320,6 +322,8 @@ struct CodecU16 {
         store_block(it, bx, by, bz, R);
     }
 };
+
+using CodecU16 = CodecU16T<false>;
 
 struct CodecU32 {
     static constexpr int kRegs = 64;
@@ -439,20 +443,18 @@ __device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len
     }
 }
 
-// Look-back window: one probe round reads the lb_win(TILE) predecessor words
-// nearest to the tile, one per thread of the first lb_win / 64 waves (whole
-// waves: a wave's unread lanes would count as empty predecessors).  Tiles are
-// resolved two iterations after they publish their aggregates, so an
-// inclusive prefix is nearly always close by; every tile pays for the window's
-// status reads (agent-scope loads of lines all workgroups are writing).
-// Timed on the CSSI volume, compression lossless / lossy: at tile 512 (uint16)
-// 512 words 0.861 / 0.858 ms, 256 0.82 / 0.81, 128 0.78-0.79 / 0.81, 64
-// 0.79 / 0.81; at tile 256 (uint32) 256 words 1.68 / 1.65, 128 1.62 / 1.61,
-// 64 1.57-1.58 / 1.59, 32 1.59 / 1.62.  A quarter of the tile, at least 64.
+// Look-back window `win` (a launch parameter, a multiple of 64 up to the
+// tile): one probe round reads the `win` predecessor words nearest to the
+// tile, one per thread of the first win / 64 waves (whole waves: a wave's
+// unread lanes would count as empty predecessors).  Tiles are resolved two
+// iterations after they publish their aggregates, so an inclusive prefix is
+// nearly always close by; every tile pays for the window's status reads
+// (agent-scope loads of lines all workgroups are writing).  Timed on the CSSI
+// volume, compression lossless / lossy: at tile 512 (uint16) 512 words 0.861 /
+// 0.858 ms, 256 0.82 / 0.81, 128 0.78-0.79 / 0.81, 64 0.79 / 0.81; at tile
+// 256 (uint32) 256 words 1.68 / 1.65, 128 1.62 / 1.61, 64 1.57-1.58 / 1.59.
+// The default is a quarter of the tile, at least 64 (default_window).
 constexpr int lb_per(int) { return 1; }   // probe words per thread
-constexpr uint32_t lb_win(int tile) { return tile >= 256 ? (uint32_t)tile / 4 : 64u; }
-static_assert(lb_win(128) % 64 == 0 && lb_win(256) % 64 == 0 && lb_win(512) % 64 == 0 && lb_win(128) <= 128,
-              "the look-back window covers whole waves of the tile");
 
 // LDS of the workgroup-wide look-back, per slice of 64 window positions (one
 // wave's k-th words): nearest inclusive and empty positions and the sum of the
@@ -475,6 +477,9 @@ struct TileShared {
     LbShared<kWaves * lb_per(TILE)> lbf;   // compression: first round, by iteration parity
     uint32_t tile, next, epoch, ntot;
     uint32_t next2[2];             // compression: next tile id, by iteration parity
+    uint32_t cls;                  // the workgroup's tile-id class
+    uint32_t tagg[64];             // decode offset phase: payload words of a chunk's tiles
+    uint32_t chunk, chunk_agg;     // decode offset phase: chunk id, chunk total
 };
 
 // One probe round: this thread's words of the predecessors at distances
@@ -483,12 +488,12 @@ struct TileShared {
 // tile) read as an inclusive prefix of 0.
 template <int TILE>
 __device__ __forceinline__ void lb_probe(const uint64_t *status, int64_t base, uint32_t first, uint32_t epoch,
-                                         uint64_t w[lb_per(TILE)])
+                                         uint32_t win, uint64_t w[lb_per(TILE)])
 {
 #pragma unroll
     for (int k = 0; k < lb_per(TILE); k++) {
         const int64_t idx = base - (int64_t)k * TILE - (int64_t)threadIdx.x;
-        const bool inwin = (uint32_t)k * TILE + threadIdx.x < lb_win(TILE);
+        const bool inwin = (uint32_t)k * TILE + threadIdx.x < win;
         w[k] = !inwin ? 0ull : idx >= (int64_t)first ? ld_relaxed(status + idx) : lb_word(epoch, LB_INCL, 0);
     }
 }
@@ -505,11 +510,10 @@ enum : uint32_t { LB_RETRY = 0, LB_NEXT = 1, LB_DONE = 2 };
 // blocks of at most 62), so the aggregate lanes of a slice are summed in 32
 // bits and the inclusive lane's value is added in 64.  No barrier.
 template <int TILE>
-__device__ __forceinline__ void lb_slices(const uint64_t w[lb_per(TILE)], uint32_t epoch,
+__device__ __forceinline__ void lb_slices(const uint64_t w[lb_per(TILE)], uint32_t epoch, uint32_t kWin,
                                           LbShared<TILE / 64 * lb_per(TILE)> &sh, uint32_t par)
 {
     constexpr int P = lb_per(TILE), NW = TILE / 64;
-    constexpr uint32_t kWin = lb_win(TILE);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
 #pragma unroll
     for (int k = 0; k < P; k++) {
@@ -543,10 +547,9 @@ __device__ __forceinline__ void lb_slices(const uint64_t w[lb_per(TILE)], uint32
 // lies before the nearest inclusive one: read the same window again).
 template <int TILE>
 __device__ __forceinline__ uint32_t lb_reduce(const LbShared<TILE / 64 * lb_per(TILE)> &sh, uint32_t par,
-                                              uint64_t &excl)
+                                              uint32_t kWin, uint64_t &excl)
 {
     constexpr int NS = TILE / 64 * lb_per(TILE);
-    constexpr uint32_t kWin = lb_win(TILE);
     uint32_t j = kWin, e = kWin;
     for (int v = 0; v < NS; v++) {
         const uint32_t x = sh.pos_incl[par][v], y = sh.pos_empty[par][v];
@@ -569,16 +572,16 @@ __device__ __forceinline__ uint32_t lb_reduce(const LbShared<TILE / 64 * lb_per(
 // *err receives the launch error word when `poll` is set (else 0).  `par`
 // alternates between consecutive calls (LDS buffer).  Called by every thread.
 template <int TILE>
-__device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint32_t epoch,
+__device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint32_t epoch, uint32_t win,
                                             LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, bool poll,
                                             uint32_t par, uint64_t &excl, uint32_t *err)
 {
-    lb_slices<TILE>(w, epoch, sh, par);
+    lb_slices<TILE>(w, epoch, win, sh, par);
     if (threadIdx.x == 0)
         sh.err[par] = poll ? __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
     __syncthreads();
     *err = sh.err[par];
-    return lb_reduce<TILE>(sh, par, excl);
+    return lb_reduce<TILE>(sh, par, win, excl);
 }
 
 // Look-back from the window ending at tile `base` (inclusive) towards the
@@ -588,20 +591,20 @@ __device__ __forceinline__ uint32_t lb_eval(const uint64_t w[lb_per(TILE)], uint
 // flags the launch (ERRF_TIMEOUT).  Called by every thread; barriers inside.
 template <int TILE>
 __device__ __forceinline__ uint64_t lookback_from(uint64_t *status, int64_t base, uint32_t first, uint32_t epoch,
-                                  LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, uint64_t excl,
+                                  uint32_t win, LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl, uint64_t excl,
                                   uint32_t spins, uint32_t par)
 {
     for (;;) {
         uint64_t w[lb_per(TILE)];
-        lb_probe<TILE>(status, base, first, epoch, w);
+        lb_probe<TILE>(status, base, first, epoch, win, w);
         // the launch-wide error word is polled only while waiting, every 64
         // probes: it must not become a hot spot for every look-back
         uint32_t err;
-        const uint32_t r = lb_eval<TILE>(w, epoch, sh, ctrl, (spins & 63u) == 63u, par, excl, &err);
+        const uint32_t r = lb_eval<TILE>(w, epoch, win, sh, ctrl, (spins & 63u) == 63u, par, excl, &err);
         par ^= 1u;
         if (r == LB_DONE) return excl;
         if (r == LB_NEXT) {
-            base -= (int64_t)lb_win(TILE);
+            base -= (int64_t)win;
             continue;
         }
         if ((err & ERRF_TIMEOUT) || ++spins > kSpinLimit) {
@@ -617,10 +620,10 @@ __device__ __forceinline__ uint64_t lookback_from(uint64_t *status, int64_t base
 // its item by decoupled look-back.
 template <int TILE>
 __device__ __forceinline__ uint64_t lookback_tile(uint64_t *status, uint32_t tile, uint32_t first,
-                                                  uint32_t epoch, LbShared<TILE / 64 * lb_per(TILE)> &sh,
-                                                  Ctrl *ctrl)
+                                                  uint32_t epoch, uint32_t win,
+                                                  LbShared<TILE / 64 * lb_per(TILE)> &sh, Ctrl *ctrl)
 {
-    return lookback_from<TILE>(status, (int64_t)tile - 1, first, epoch, sh, ctrl, 0, 0, 0);
+    return lookback_from<TILE>(status, (int64_t)tile - 1, first, epoch, win, sh, ctrl, 0, 0, 0);
 }
 
 __device__ __forceinline__ void block_coords(const Item &it, uint32_t b, uint32_t &bx,
@@ -639,7 +642,11 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
         const uint32_t old = __hip_atomic_fetch_add(&ctrl->done_ctr, 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
         if (old == gridDim.x - 1) {
-            __hip_atomic_store(&ctrl->tile_ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctrl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctrl->chunk.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (uint32_t c = 0; c < kMaxCounters; c++)
+                __hip_atomic_store(&ctrl->ctr[c].v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctrl->done_ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctrl->epoch, (epoch + 1u) & kEpochMask, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -647,22 +654,37 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
     }
 }
 
-// Dynamic tile ids.  first_tile() takes an id for immediate processing.  Only
-// where no tile waits for another (decode with precomputed offsets) are ids
-// taken ahead (prefetch_tile() and the decode loop): a reserved tile must never
-// hold up a look-back.
-template <class SH>
-__device__ __forceinline__ void first_tile(SH &sh, Ctrl *ctrl)
+// Dynamic tile ids (thread 0 of a workgroup only).  Tile k*nc + c is the k-th
+// id of counter c; a workgroup's class c is its ticket (its place in the start
+// order) mod nc, taken once, so every class has a running member as soon as nc
+// workgroups run, and a tile id is handed only to a running workgroup of its
+// class.  Forward progress: the lowest tile without an inclusive prefix is
+// either held by a running workgroup, whose look-backs all wait for lower
+// tiles only, or not yet taken, and then no member of its class waits (their
+// pending tiles are lower), so one of them takes it.  Only where no tile
+// waits for another (decode with precomputed offsets) are ids taken ahead: a
+// reserved tile must never hold up a look-back.
+struct TileQ {
+    uint32_t cls, nc;
+    __device__ __forceinline__ uint32_t take(Ctrl *ctrl) const
+    {
+        const uint32_t k = __hip_atomic_fetch_add(&ctrl->ctr[cls].v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return k * nc + cls;
+    }
+};
+
+// Thread 0: the workgroup's class (one ticket atomic when there are several
+// counters).  It is broadcast through LDS so that every thread holds the
+// queue in scalar registers.
+__device__ __forceinline__ uint32_t tile_class(Ctrl *ctrl, Sched sc)
 {
-    if (threadIdx.x == 0)
-        sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (sc.nc <= 1) return 0;
+    return __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) % sc.nc;
 }
 
-template <class SH>
-__device__ __forceinline__ void prefetch_tile(SH &sh, Ctrl *ctrl)
+__device__ __forceinline__ TileQ uniform_queue(uint32_t cls, Sched sc)
 {
-    if (threadIdx.x == 0)
-        sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return TileQ{(uint32_t)__builtin_amdgcn_readfirstlane(cls), (uint32_t)__builtin_amdgcn_readfirstlane(sc.nc)};
 }
 
 // Item of a global tile: the single item, or items[tile_item[tile]].
@@ -766,18 +788,22 @@ __device__ __forceinline__ void store_tile(const Item &it, uint32_t tl, uint32_t
 template <class C, int TILE, bool kTable>
 __device__ __forceinline__ void compress_tiles(const Item single, const Item *__restrict__ items,
                                                const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
-                                               const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
+                                               const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status,
+                                               Sched sc)
 {
     constexpr uint32_t kRing = (uint32_t)TILE * C::kMaxWords;   // staging words: one worst-case tile
     __shared__ TileShared<TILE, C::kMaxWords> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t win = sc.win;
     if (tid == 0) {
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
-        sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh.cls = tile_class(ctrl, sc);
+        sh.next = TileQ{sh.cls, sc.nc}.take(ctrl);
     }
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
+    const TileQ q = uniform_queue(sh.cls, sc);
 
     uint32_t np = 0, hb = 0;   // pending tiles (p0 oldest), header buffer of the next tile
     PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
@@ -805,12 +831,13 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
         }
         const bool p_lb = resolve && p0.tl != 0;   // p0 needs a look-back
         uint64_t w0[lb_per(TILE)];
-        if (p_lb) lb_probe<TILE>(status, (int64_t)p0.tile - 1, item_at<kTable>(single, items, p0.item).first_tile, epoch, w0);
+        if (p_lb)
+            lb_probe<TILE>(status, (int64_t)p0.tile - 1, item_at<kTable>(single, items, p0.item).first_tile, epoch,
+                           win, w0);
         // Issued last: returns are counted in order, so waiting for the loads or
         // the probe must not include the atomic's longer round trip.
         uint32_t next_id = n;
-        if (tid == 0 && have_n)
-            next_id = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0 && have_n) next_id = q.take(ctrl);
 
         // 2. header of n; n's scan and the first half of p0's evaluation share a barrier
         uint32_t n_hdr = 0, n_words = 0;
@@ -823,7 +850,7 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
         }
         const uint32_t incl = wave_incl_scan(n_words, lane);
         if (lane == 63) sh.wsum[par][wave] = incl;
-        if (p_lb) lb_slices<TILE>(w0, epoch, sh.lbf, par);
+        if (p_lb) lb_slices<TILE>(w0, epoch, win, sh.lbf, par);
         if (tid == 0) sh.next2[par] = next_id;
         __syncthreads();
         const uint32_t n_next = sh.next2[par];
@@ -847,10 +874,10 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
             const Item &pit = item_at<kTable>(single, items, p0.item);
             uint64_t excl = 0;
             if (p_lb) {
-                const uint32_t r = lb_reduce<TILE>(sh.lbf, par, excl);
+                const uint32_t r = lb_reduce<TILE>(sh.lbf, par, win, excl);
                 if (r != LB_DONE) {
-                    const int64_t base = (int64_t)p0.tile - 1 - (r == LB_NEXT ? (int64_t)lb_win(TILE) : 0);
-                    excl = lookback_from<TILE>(status, base, pit.first_tile, epoch, sh.lb, ctrl, excl,
+                    const int64_t base = (int64_t)p0.tile - 1 - (r == LB_NEXT ? (int64_t)win : 0);
+                    excl = lookback_from<TILE>(status, base, pit.first_tile, epoch, win, sh.lb, ctrl, excl,
                                                r == LB_RETRY ? 1u : 0u, 0u);
                 }
                 if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
@@ -868,7 +895,7 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
                 const Item &pit = item_at<kTable>(single, items, p0.item);
                 uint64_t excl = 0;
                 if (p0.tl != 0) {
-                    excl = lookback_tile<TILE>(status, p0.tile, pit.first_tile, epoch, sh.lb, ctrl);
+                    excl = lookback_tile<TILE>(status, p0.tile, pit.first_tile, epoch, win, sh.lb, ctrl);
                     if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
                 }
                 store_tile<TILE>(pit, p0.tl, p0.agg, sh.hdr[p0.hb], sh.pay + p0.start, excl, tid);
@@ -893,20 +920,134 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
 template <int TILE, bool kTable>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(
     const Item single, const Item *__restrict__ items, const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
-    const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status)
+    const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status, Sched sc)
 {
-    compress_tiles<CodecU16, TILE, kTable>(single, items, tile_items, total_tiles, d_total_tiles, ctrl, status);
+    compress_tiles<CodecU16T<kTable>, TILE, kTable>(single, items, tile_items, total_tiles, d_total_tiles, ctrl, status,
+                                                    sc);
 }
 
 // ---------------------------------------------------------------------------
 // Decompression.
 // ---------------------------------------------------------------------------
+
+// payload words of the 4 headers packed in one dword
+__device__ __forceinline__ uint32_t words4(uint32_t x)
+{
+    const uint32_t rate = x & 0x1f1f1f1fu;
+    const uint32_t plain = ((~x >> 7) & 0x01010101u) * 0x1fu;   // 0x1f in bytes of plain blocks
+    const uint32_t w = rate + (rate & plain);                     // rate * (plain ? 2 : 1), <= 62 per byte
+    return (w * 0x01010101u) >> 24;                               // byte sum <= 248
+}
+
+// Payload words of the 16 header bytes at [off, off + 16) of an item; bytes at
+// or past the last block count nothing (the last piece reads dwords below R).
+__device__ __forceinline__ uint32_t words16(const Item &it, uint64_t off)
+{
+    if (off >= it.nblocks) return 0;
+    uint4 x;
+    if (off + 16 <= it.nblocks) {
+        x = *(const uint4 *)(it.s_in + off);
+    } else {
+        const uint32_t *s32 = (const uint32_t *)(it.s_in + off);
+        uint32_t t[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const uint64_t at = off + 4u * (uint32_t)d;
+            if (at < it.nblocks) {
+                const uint64_t keep = it.nblocks - at;
+                t[d] = keep < 4 ? s32[d] & (0xffffffffu >> (32 - 8 * (uint32_t)keep)) : s32[d];
+            }
+        }
+        x = make_uint4(t[0], t[1], t[2], t[3]);
+    }
+    return words4(x.x) + words4(x.y) + words4(x.z) + words4(x.w);
+}
+
+// Payload offset (in words after the header section) of tile u of a single
+// item: the word the offset phase or the pre-pass wrote, tagged with this
+// launch's epoch and the inclusive flag (a cleared word, epoch 0 and flag
+// empty, never passes).  Waits, bounded, while the word is not yet of this
+// launch (its chunk is still being resolved); on a timeout the launch is
+// flagged and an offset past any stream is returned, so the tile decodes to
+// zeros.
+__device__ __forceinline__ uint64_t tile_offset(const uint64_t *tile_off, uint32_t u, uint32_t epoch, Ctrl *ctrl)
+{
+    const uint32_t want = (epoch << 2) | LB_INCL;   // bits 36.. of the word
+    uint64_t w = ld_relaxed(tile_off + u);
+    for (uint32_t spins = 1; (uint32_t)(w >> 36) != want; spins++) {
+        if (spins > kSpinLimit ||
+            ((spins & 63u) == 0 &&
+             (__hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERRF_TIMEOUT))) {
+            __hip_atomic_fetch_or(&ctrl->error, ERRF_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return ~0ull >> 3;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w = ld_relaxed(tile_off + u);
+    }
+    return w & kValMask;
+}
+
+// First phase of a single-volume decode launch: the payload offset of every
+// tile, from the header bytes alone (1/128 of the volume).  Chunks of 64 tiles
+// come from their own counter, so every chunk is taken before any workgroup
+// starts decoding.  TILE/64 threads sum 4 16-byte header pieces each of one
+// tile (each load instruction of a wave reads TILE/4 contiguous header bytes
+// of each of 4096/TILE tiles), the tile sums of a chunk are scanned, chunks
+// are chained by the look-back (one status word per chunk), and tile_off[t] =
+// lb_word(epoch, LB_INCL, offset of tile t).  Called by every thread;
+// barriers inside.
+template <int TILE, class SH>
+__device__ __forceinline__ void offset_phase(const Item &it, Ctrl *ctrl, uint64_t *status, uint64_t *tile_off,
+                                             uint32_t epoch, uint32_t win, SH &sh)
+{
+    constexpr uint32_t kPer = TILE / 64;   // threads per tile; 4 pieces each cover its TILE header bytes
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t nchunks = (it.ntiles + 63) / 64;
+    if (tid == 0) sh.chunk = __hip_atomic_fetch_add(&ctrl->chunk.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    for (;;) {
+        const uint32_t chunk = sh.chunk;
+        if (chunk >= nchunks) break;
+        const uint32_t j = tid / kPer, q = tid % kPer;
+        const uint64_t tbase = ((uint64_t)chunk * 64 + j) * TILE;   // the tile's first header byte
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) sum += words16(it, tbase + 16ull * (q + k * kPer));
+#pragma unroll
+        for (uint32_t d = 1; d < kPer; d <<= 1) sum += __shfl_xor(sum, (int)d, 64);
+        if (q == 0) sh.tagg[j] = sum;
+        __syncthreads();
+        uint32_t incl = 0;
+        if (tid < 64) {
+            incl = wave_incl_scan(sh.tagg[lane], lane);
+            if (lane == 63) {
+                sh.chunk_agg = incl;
+                st_relaxed(status + chunk, lb_word(epoch, chunk == 0 ? LB_INCL : LB_AGG, incl));
+            }
+        }
+        __syncthreads();
+        uint64_t excl = 0;
+        if (chunk != 0) {
+            excl = lookback_tile<TILE>(status, chunk, 0, epoch, win, sh.lb, ctrl);
+            if (tid == 0) st_relaxed(status + chunk, lb_word(epoch, LB_INCL, excl + sh.chunk_agg));
+        }
+        if (tid < 64) {
+            const uint32_t t = chunk * 64 + lane;
+            if (t < it.ntiles) st_relaxed(tile_off + t, lb_word(epoch, LB_INCL, excl + incl - sh.tagg[lane]));
+        }
+        __syncthreads();
+        if (tid == 0)
+            sh.chunk = __hip_atomic_fetch_add(&ctrl->chunk.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
+}
+
 template <int TILE, bool kTable>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_kernel(const Item single, const Item *__restrict__ items,
                                                           const uint32_t *__restrict__ tile_items,
                                                           uint32_t total_tiles, const uint32_t *d_total_tiles,
                                                           Ctrl *ctrl, uint64_t *status,
-                                                          const uint64_t *__restrict__ tile_off)
+                                                          uint64_t *__restrict__ tile_off, Sched sc, uint32_t phase)
 {
     __shared__ TileShared<TILE> sh;
     const uint32_t tid = threadIdx.x;
@@ -914,10 +1055,17 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
     }
-    first_tile(sh, ctrl);
-    if (tile_off) prefetch_tile(sh, ctrl);
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
+    if (!kTable && phase) offset_phase<TILE>(single, ctrl, status, tile_off, epoch, sc.win, sh);
+    if (tid == 0) {
+        sh.cls = tile_class(ctrl, sc);
+        const TileQ q0{sh.cls, sc.nc};
+        sh.tile = q0.take(ctrl);
+        if (tile_off) sh.next = q0.take(ctrl);
+    }
+    __syncthreads();
+    const TileQ q = uniform_queue(sh.cls, sc);
 
     // With precomputed tile offsets (no look-back) the header byte and the
     // offset of the next tile are loaded while the current tile decodes, and
@@ -931,7 +1079,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
         const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
         hdr = b < it.nblocks ? (uint32_t)it.s_in[b] : 0u;
-        excl_next = tile_off[sh.tile];
+        excl_next = tile_offset(tile_off, sh.tile, epoch, ctrl);
     }
 
     for (;;) {
@@ -939,8 +1087,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         if (tile >= ntot) break;
         const uint32_t ntile = tile_off ? sh.next : ntot;
         uint32_t after_next = 0;   // thread 0: the id two tiles ahead
-        if (tile_off && tid == 0)
-            after_next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tile_off && tid == 0) after_next = q.take(ctrl);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
         const uint32_t b = tl * (uint32_t)TILE + tid;
@@ -957,7 +1104,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
             excl = 0;
             if (tl != 0) {
-                excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sh.lb, ctrl);
+                excl = lookback_tile<TILE>(status, tile, it.first_tile, epoch, sc.win, sh.lb, ctrl);
                 if (tid == 0) st_relaxed(status + tile, lb_word(epoch, LB_INCL, excl + agg));
             }
         }
@@ -972,7 +1119,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
             const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
             hdr = nb < nit.nblocks ? (uint32_t)nit.s_in[nb] : 0u;
-            if (tile_off) excl_next = tile_off[ntile];
+            if (tile_off) excl_next = tile_offset(tile_off, ntile, epoch, ctrl);
         }
         __syncthreads();
 
@@ -992,13 +1139,13 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                                            __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        if (tile_off) {
-            if (tid == 0) {
+        if (tid == 0) {
+            if (tile_off) {
                 sh.tile = ntile;
                 sh.next = after_next;
+            } else {
+                sh.tile = q.take(ctrl);
             }
-        } else {
-            first_tile(sh, ctrl);
         }
         __syncthreads();
     }
@@ -1022,22 +1169,28 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // payload is a few hundred bytes, the decode with the payload loaded in its own
 // iteration was 2-5 % faster in timings.
 template <class C, int TILE>
-__device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
+__device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl, uint64_t *status,
+                                                      uint64_t *__restrict__ tile_off, Sched sc, uint32_t phase)
 {
     constexpr uint32_t kMax = (uint32_t)TILE * C::kMaxWords;   // payload words of a tile at most
     constexpr uint32_t kRing = kMax + 8u;                     // staging words (sh.pay)
     __shared__ TileShared<TILE, C::kMaxWords> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t ntot = it.ntiles;
-    uint32_t pend = 0;   // thread 0: the id of the tile two ahead of the next one
-    if (tid == 0) {
-        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh.tile = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh.next = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        pend = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (tid == 0) sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const uint32_t epoch = sh.epoch;
+    if (phase) offset_phase<TILE>(it, ctrl, status, tile_off, epoch, sc.win, sh);
+    uint32_t pend = 0;   // thread 0: the id of the tile two ahead of the next one
+    if (tid == 0) {
+        sh.cls = tile_class(ctrl, sc);
+        const TileQ q0{sh.cls, sc.nc};
+        sh.tile = q0.take(ctrl);
+        sh.next = q0.take(ctrl);
+        pend = q0.take(ctrl);
+    }
+    __syncthreads();
+    const TileQ q = uniform_queue(sh.cls, sc);
     uint32_t t = sh.tile, t1 = sh.next;
 
     // header byte of this thread's block in tile u, and u's payload byte offset
@@ -1046,7 +1199,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
         return (u < ntot && b < it.nblocks) ? (uint32_t)it.s_in[b] : 0u;
     };
     const auto g0_of = [&](uint32_t u) -> uint64_t {
-        return u < ntot ? (uint64_t)it.hdr_bytes + 4ull * tile_off[u] : 0ull;
+        return u < ntot ? (uint64_t)it.hdr_bytes + 4ull * tile_offset(tile_off, u, epoch, ctrl) : 0ull;
     };
 
     // tile t: scan, then its payload staged at the start of the ring
@@ -1074,8 +1227,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
             agg1 += x;
         }
         const uint32_t off1 = before + incl - words1;
-        if (tid == 0)
-            pend = __hip_atomic_fetch_add(&ctrl->tile_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) pend = q.take(ctrl);
 
         // 2. t2's header byte and offset, t1's payload loads (in this order: the
         //    wait for the payload in step 4 then covers all of them)
@@ -1150,9 +1302,9 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
 
 template <int TILE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_pipe_kernel(
-    const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
+    const Item it, Ctrl *ctrl, uint64_t *status, uint64_t *__restrict__ tile_off, Sched sc, uint32_t phase)
 {
-    decompress_pipe_tiles<CodecU16, TILE>(it, ctrl, tile_off);
+    decompress_pipe_tiles<CodecU16, TILE>(it, ctrl, status, tile_off, sc, phase);
 }
 
 // ---------------------------------------------------------------------------
@@ -1161,8 +1313,9 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // (4 threads per tile, TILE/4 header bytes each, read as 16-byte pieces so
 // that a wave's load covers 64 contiguous bytes of each of 16 tiles), reduces
 // sizes per tile, scans them, and chains chunks with the same look-back.
-// tile_off[t] is the word offset of tile t's payload after the header section;
-// tile_off[ntiles] is the total.
+// tile_off[t] is the word offset of tile t's payload after the header
+// section, tagged with the epoch of the decode launch that follows (this
+// launch's epoch + 1: its last workgroup advances the epoch by one).
 // ---------------------------------------------------------------------------
 struct OffShared {
     uint32_t tagg[64];
@@ -1171,15 +1324,6 @@ struct OffShared {
     uint32_t excl_lo, excl_hi;
 };
 
-// payload words of the 4 headers packed in one dword
-__device__ __forceinline__ uint32_t words4(uint32_t x)
-{
-    const uint32_t rate = x & 0x1f1f1f1fu;
-    const uint32_t plain = ((~x >> 7) & 0x01010101u) * 0x1fu;   // 0x1f in bytes of plain blocks
-    const uint32_t w = rate + (rate & plain);                     // rate * (plain ? 2 : 1), <= 62 per byte
-    return (w * 0x01010101u) >> 24;                               // byte sum <= 248
-}
-
 template <int TILE>
 __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl, uint64_t *status,
                                                      uint64_t *__restrict__ tile_off)
@@ -1187,8 +1331,11 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
     __shared__ OffShared sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t nchunks = (it.ntiles + 63) / 64;
-    if (tid == 0) sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    first_tile(sh, ctrl);
+    const TileQ tq = {0u, 1u};   // one counter: chunks are few
+    if (tid == 0) {
+        sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh.tile = tq.take(ctrl);
+    }
     __syncthreads();
     const uint32_t epoch = sh.epoch;
     constexpr uint32_t kBytes = TILE / 4;   // header bytes per thread
@@ -1242,16 +1389,16 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
         const uint32_t cagg = sh.excl_lo;
         uint64_t excl = 0;
         if (chunk != 0) {
-            excl = lookback_tile<256>(status, chunk, 0, epoch, sh.lb, ctrl);
+            excl = lookback_tile<256>(status, chunk, 0, epoch, default_window(256), sh.lb, ctrl);
             if (tid == 0) st_relaxed(status + chunk, lb_word(epoch, LB_INCL, excl + cagg));
         }
         if (tid < 64) {
             const uint32_t t = chunk * 64 + lane;
-            if (t < it.ntiles) tile_off[t] = excl + incl - sh.tagg[lane];
-            if (t == it.ntiles - 1) tile_off[it.ntiles] = excl + incl;
+            if (t < it.ntiles)
+                st_relaxed(tile_off + t, lb_word((epoch + 1u) & kEpochMask, LB_INCL, excl + incl - sh.tagg[lane]));
         }
         __syncthreads();
-        first_tile(sh, ctrl);
+        if (tid == 0) sh.tile = tq.take(ctrl);
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
@@ -1262,6 +1409,15 @@ __global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t value)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = value;
+}
+
+// Pre-pass kernel for the tile offsets of a single-item decode (OFF_KERNEL).
+template <int TILE>
+static void launch_offsets(const Item &it, const LaunchCfg &cfg, Ctrl *ctrl, uint64_t *status, hipStream_t stream)
+{
+    const uint32_t nchunks = (it.ntiles + 63) / 64;
+    const uint32_t g = nchunks < cfg.grid ? nchunks : cfg.grid;
+    hipLaunchKernelGGL((offsets_kernel<TILE>), dim3(g), dim3(256), 0, stream, it, ctrl, status, cfg.tile_off);
 }
 
 // The pipelined decode pays off once payloads are dense: measured on one
@@ -1278,32 +1434,29 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
                            Ctrl *ctrl, uint64_t *status, hipStream_t stream)
 {
     const dim3 grid(cfg.grid), block(TILE);
+    const Sched sc = cfg.sched;
     if (compress) {
         if (items)
             hipLaunchKernelGGL((compress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
-                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status, sc);
         else
             hipLaunchKernelGGL((compress_kernel<TILE, false>), grid, block, 0, stream, single, items, tile_items,
-                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status);
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status, sc);
     } else {
         if (items)
             hipLaunchKernelGGL((decompress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
-                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status, (const uint64_t *)nullptr);
+                               cfg.total_tiles, cfg.d_total_tiles, ctrl, status, (uint64_t *)nullptr, sc, 0u);
         else
         {
-            if (cfg.tile_off) {
-                const uint32_t nchunks = (single.ntiles + 63) / 64;
-                const uint32_t g = nchunks < cfg.grid ? nchunks : cfg.grid;
-                hipLaunchKernelGGL((offsets_kernel<TILE>), dim3(g), dim3(256), 0, stream, single, ctrl, status,
-                                   cfg.tile_off);
-            }
+            const uint32_t phase = cfg.tile_off && cfg.off_mode == OFF_PHASE ? 1u : 0u;
+            if (cfg.tile_off && !phase) launch_offsets<TILE>(single, cfg, ctrl, status, stream);
             if (cfg.tile_off && use_pipe_decode(single))
-                hipLaunchKernelGGL((decompress_pipe_kernel<TILE>), grid, block, 0, stream, single, ctrl,
-                                   (const uint64_t *)cfg.tile_off);
+                hipLaunchKernelGGL((decompress_pipe_kernel<TILE>), grid, block, 0, stream, single, ctrl, status,
+                                   cfg.tile_off, sc, phase);
             else
                 hipLaunchKernelGGL((decompress_kernel<TILE, false>), grid, block, 0, stream, single, items,
-                                   tile_items, cfg.total_tiles, cfg.d_total_tiles, ctrl, status,
-                                   (const uint64_t *)cfg.tile_off);
+                                   tile_items, cfg.total_tiles, cfg.d_total_tiles, ctrl, status, cfg.tile_off, sc,
+                                   phase);
         }
     }
     return hipGetLastError();
@@ -1322,16 +1475,16 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
 // ---------------------------------------------------------------------------
 template <int TILE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void compress_u32_kernel(
-    const Item it, Ctrl *ctrl, uint64_t *status)
+    const Item it, Ctrl *ctrl, uint64_t *status, Sched sc)
 {
-    compress_tiles<CodecU32, TILE, false>(it, nullptr, nullptr, it.ntiles, nullptr, ctrl, status);
+    compress_tiles<CodecU32, TILE, false>(it, nullptr, nullptr, it.ntiles, nullptr, ctrl, status, sc);
 }
 
 template <int TILE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void decompress_u32_pipe_kernel(
-    const Item it, Ctrl *ctrl, const uint64_t *__restrict__ tile_off)
+    const Item it, Ctrl *ctrl, uint64_t *status, uint64_t *__restrict__ tile_off, Sched sc, uint32_t phase)
 {
-    decompress_pipe_tiles<CodecU32, TILE>(it, ctrl, tile_off);
+    decompress_pipe_tiles<CodecU32, TILE>(it, ctrl, status, tile_off, sc, phase);
 }
 
 hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,
@@ -1340,14 +1493,13 @@ hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, 
     constexpr int T = (int)kTileU32;
     const dim3 grid(cfg.grid), block(T);
     if (compress) {
-        hipLaunchKernelGGL((compress_u32_kernel<T>), grid, block, 0, stream, it, ctrl, status);
+        hipLaunchKernelGGL((compress_u32_kernel<T>), grid, block, 0, stream, it, ctrl, status, cfg.sched);
     } else {
         if (!cfg.tile_off) return hipErrorInvalidValue;
-        const uint32_t nchunks = (it.ntiles + 63) / 64;
-        const uint32_t g = nchunks < cfg.grid ? nchunks : cfg.grid;
-        hipLaunchKernelGGL((offsets_kernel<T>), dim3(g), dim3(256), 0, stream, it, ctrl, status, cfg.tile_off);
-        hipLaunchKernelGGL((decompress_u32_pipe_kernel<T>), grid, block, 0, stream, it, ctrl,
-                           (const uint64_t *)cfg.tile_off);
+        const uint32_t phase = cfg.off_mode == OFF_PHASE ? 1u : 0u;
+        if (!phase) launch_offsets<T>(it, cfg, ctrl, status, stream);
+        hipLaunchKernelGGL((decompress_u32_pipe_kernel<T>), grid, block, 0, stream, it, ctrl, status, cfg.tile_off,
+                           cfg.sched, phase);
     }
     return hipGetLastError();
 }

@@ -78,7 +78,14 @@ EXPORTS = (
     "lscomp_max_compressed_bytes_u32", "lscomp_compress_u32", "lscomp_compress_u32_async",
     "lscomp_decompress_u32", "lscomp_decompress_u32_async",
     "lscomp_container_bytes", "lscomp_container_pack", "lscomp_container_parse",
+    "lscomp_set_launch_shape_ex", "lscomp_get_launch_shape_ex",
 )
+
+
+class LaunchShape(ctypes.Structure):
+    """lscomp_launch_shape."""
+    _fields_ = [("tile", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("lb_window", ctypes.c_uint32),
+                ("counters", ctypes.c_uint32)]
 
 
 class ErrorStats(ctypes.Structure):
@@ -162,6 +169,10 @@ def _load():
     lib.lscomp_get_launch_shape.restype = ctypes.c_int
     lib.lscomp_get_launch_shape.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(ctypes.c_uint32)]
+    lib.lscomp_set_launch_shape_ex.restype = ctypes.c_int
+    lib.lscomp_set_launch_shape_ex.argtypes = [ctypes.c_int, ctypes.POINTER(LaunchShape)]
+    lib.lscomp_get_launch_shape_ex.restype = ctypes.c_int
+    lib.lscomp_get_launch_shape_ex.argtypes = [ctypes.c_int, ctypes.POINTER(LaunchShape)]
     lib.lscomp_plan_create.restype = ctypes.c_int
     lib.lscomp_plan_create.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(vp)]
     lib.lscomp_plan_set_items_u16.restype = ctypes.c_int
@@ -469,6 +480,23 @@ def get_launch_shape(direction: int):
     return int(t.value), int(w.value)
 
 
+def set_launch_shape_ex(direction: int, tile: int = 0, wg_per_cu: int = 0, lb_window: int = 0,
+                        counters: int = 0) -> None:
+    """The full launch shape of a direction (0 = default for each field)."""
+    sh = LaunchShape(tile, wg_per_cu, lb_window, counters)
+    st = _lib.lscomp_set_launch_shape_ex(direction, ctypes.byref(sh))
+    if st != OK:
+        raise LscompError(st, "lscomp_set_launch_shape_ex")
+
+
+def get_launch_shape_ex(direction: int) -> dict:
+    sh = LaunchShape()
+    st = _lib.lscomp_get_launch_shape_ex(direction, ctypes.byref(sh))
+    if st != OK:
+        raise LscompError(st, "lscomp_get_launch_shape_ex")
+    return {"tile": sh.tile, "wg_per_cu": sh.wg_per_cu, "lb_window": sh.lb_window, "counters": sh.counters}
+
+
 def autotune(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, reps: int = 3, stream=None):
     """Time every launch shape on this volume, apply the fastest per direction.
 
@@ -514,9 +542,19 @@ def copy_bytes(dst, src, stream=None) -> None:
         raise LscompError(st, "lscomp_copy_bytes")
 
 
-def set_decode_prepass(enable: bool) -> bool:
-    """Single-volume decode: tile offsets by pre-pass (True, default) or look-back; returns the old setting."""
-    return bool(_lib.lscomp_set_decode_prepass(1 if enable else 0))
+DECODE_LOOKBACK, DECODE_PHASE, DECODE_PREPASS = 0, 1, 2
+
+
+def set_decode_prepass(mode) -> int:
+    """Single-volume decode: tile offsets from a first phase of the decode launch
+    (DECODE_PHASE, default; True means this too), a separate pre-pass kernel
+    (DECODE_PREPASS) or a look-back inside the decode (DECODE_LOOKBACK; False
+    means this too).  Returns the old mode."""
+    m = DECODE_PHASE if mode is True else DECODE_LOOKBACK if mode is False else int(mode)
+    old = int(_lib.lscomp_set_decode_prepass(m))
+    if old < 0:
+        raise LscompError(-old, "lscomp_set_decode_prepass")
+    return old
 
 
 def version() -> int:

@@ -78,6 +78,10 @@ def block_host():
     lib.bh_quant_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     lib.bh_quant2_mismatches.restype = ctypes.c_uint64
     lib.bh_quant2_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    lib.bh_q24_mismatches.restype = ctypes.c_uint64
+    lib.bh_q24_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    lib.bh_decide_flags.restype = ctypes.c_uint32
+    lib.bh_decide_flags.argtypes = [u32p, ctypes.c_float]
     lib.bh_zth.restype = ctypes.c_uint32
     lib.bh_zth.argtypes = [ctypes.c_float]
     return lib

@@ -115,6 +115,44 @@ uint64_t bh_quant2_mismatches(uint32_t d_lo, uint32_t d_hi)
     return bad;
 }
 
+// The 24-bit multiply-high division as the kernels run it (quantize_plain24 /
+// quantize_pooled24 on packed pairs).  Counts mismatches against exact
+// division over every plain value v < 2^16 (divisor d) and every pair sum
+// s < 2^17 (divisor 2d), for each bin d in [d_lo, d_hi]; a bin for which
+// make_decide does not choose the multiply-high path counts 2^40.
+uint64_t bh_q24_mismatches(uint32_t d_lo, uint32_t d_hi)
+{
+    uint64_t bad = 0;
+    for (uint64_t d = d_lo; d <= d_hi; d++) {
+        const uint32_t b4[4] = {(uint32_t)d, (uint32_t)d, (uint32_t)d, (uint32_t)d};
+        const Decide dc = make_decide(b4, 1.0f);
+        if (!(dc.flags & DEC_Q24)) {
+            bad += 1ull << 40;
+            continue;
+        }
+        uint32_t D[32], Q[32];
+        for (uint32_t v0 = 0; v0 < 65536; v0 += 32) {   // plain: v in the low half, 65535 - v in the high half
+            for (int i = 0; i < 32; i++) D[i] = (v0 + i) | ((65535u - (v0 + i)) << 16);
+            quantize_plain24(D, dc.qs[0], dc.qm[0]);
+            for (int i = 0; i < 32; i++) {
+                bad += (D[i] & 0xffffu) != (v0 + i) / d;
+                bad += (D[i] >> 16) != (65535u - (v0 + i)) / d;
+            }
+        }
+        for (uint32_t s0 = 0; s0 < (1u << 17); s0 += 32) {   // pooled: every pair sum s <= 2 * 65535
+            for (int i = 0; i < 32; i++) {                    // split into two values (the last lanes repeat s0)
+                const uint32_t s = s0 + i <= 131070u ? s0 + i : s0, a = s < 65535u ? s : 65535u;
+                D[i] = a | ((s - a) << 16);
+            }
+            quantize_pooled24(D, dc.ps[0], dc.pm[0], Q);
+            for (int i = 0; i < 32; i++) bad += Q[i] != ((s0 + i <= 131070u ? s0 + i : s0) / (2 * d));
+        }
+    }
+    return bad;
+}
+
+uint32_t bh_decide_flags(const uint32_t bins[4], float p) { return make_decide(bins, p).flags; }
+
 uint32_t bh_zth(float p)
 {
     const uint32_t b4[4] = {1, 1, 1, 1};

@@ -109,6 +109,46 @@ def test_quantize_pooled_sums(block_host):
     assert block_host.bh_quant2_mismatches(4294967000, 4294967295) == 0
 
 
+def test_quantize_mulhi24_exhaustive(block_host):
+    """The 24-bit multiply-high division (every bin in [2, 16384]) equals exact
+    division for every plain value < 2^16 and every pooled pair sum < 2^17, run
+    through the kernels' own quantize_plain24 / quantize_pooled24."""
+    assert block_host.bh_q24_mismatches(2, 16384) == 0
+
+
+def test_mulhi24_path_selection(block_host):
+    """Bins in [2, 16384] take the multiply-high path; a bin of 1 or above 16384 keeps the float path."""
+    q24 = 8
+    for bins, on in [((3, 5, 10, 15), True), ((2, 2, 2, 2), True), ((16384,) * 4, True), ((3, 4, 5, 6), True),
+                     ((1, 2, 4, 8), False), ((1, 1, 1, 1), False), ((2, 65535, 70000, 100000), False),
+                     ((16385,) * 4, False)]:
+        b = np.array(bins, np.uint32)
+        flags = block_host.bh_decide_flags(b.ctypes.data_as(u32p), ctypes.c_float(0.5))
+        assert bool(flags & q24) == on, bins
+
+
+def test_header_bin_choice_edges(block_host, orc):
+    """Blocks whose maximum sits at, just below and just above each bin edge: the
+    bin choice skips its minimum search exactly when the maximum is below the
+    second bin, and must still agree with the oracle's counts."""
+    rng = np.random.default_rng(11)
+    for bins in [(3, 5, 10, 15), (3, 3, 10, 15), (2, 65535, 70000, 100000), (1, 2, 4, 8), (5, 5, 5, 6)]:
+        b = np.array(bins, np.uint32)
+        edges = sorted({max(0, min(65535, e + k)) for e in bins for k in (-1, 0, 1)})
+        for p in (0.0, 0.5, 1.0):
+            for top in edges:
+                for low in (0, 1, bins[0] - 1 if bins[0] > 0 else 0, bins[0]):
+                    v = np.minimum(rng.integers(0, low + 1, 64), 65535).astype(np.uint16)
+                    v[int(rng.integers(0, 64))] = top
+                    h, _ = orc.block_header(v.astype(np.uint32), b, p)
+                    out = np.zeros(128, np.uint8)
+                    hh = ctypes.c_uint32(0)
+                    n = block_host.bh_encode(v.ctypes.data_as(u16p), b.ctypes.data_as(u32p), ctypes.c_float(p),
+                                             out.ctypes.data_as(u8p), ctypes.byref(hh))
+                    assert hh.value == h, (bins, p, top, low, hex(h), hex(hh.value))
+                    assert out[:n].tobytes() == orc.encode_block(v.astype(np.uint32), h, b)
+
+
 def _blocks_u32(rng, n):
     out = []
     for i in range(n):

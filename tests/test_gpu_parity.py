@@ -207,9 +207,10 @@ def test_autotune_applies_a_shape_and_keeps_bytes(gpu, orc):
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("prepass", [False, True])
+@pytest.mark.parametrize("prepass", [0, 1, 2])
 def test_decode_with_and_without_prepass(gpu, orc, prepass):
-    """Decode tile offsets from the header pre-pass or from the in-kernel look-back: same volume."""
+    """Decode tile offsets from the in-kernel look-back (0), the offset phase of
+    the decode launch (1) or a separate pre-pass kernel (2): same volume."""
     old = gpu.set_decode_prepass(prepass)
     try:
         for shape, kind in (((3, 300, 1558), "cssi"), ((1, 8, 9), "uniform"), ((64, 128, 128), "xpcs"),
