@@ -193,49 +193,75 @@ int lscomp_plan_destroy(lscomp_plan plan);
 
 /* ---- Served form --------------------------------------------------------
  * A compression server with continuous batching.  Requests (device frames)
- * are queued by lscomp_server_submit_u16; a scheduler thread admits them in
- * arrival order while their worst-case output fits into an HBM arena of
- * `budget_bytes`, and packs every admitted request into the next batch of
- * one of `slots` plans, each on its own stream, so batches started while
- * others still run overlap on the device.  Each request's stream is exactly
- * lscomp_compress_u16 of its frame alone.  Output buffers stay reserved until
- * lscomp_server_release.  No counterpart in the reference. */
+ * are queued by lscomp_server_submit_u16 and admitted in arrival order while
+ * their worst-case output fits into an HBM arena of `budget_bytes`; each
+ * request's stream is exactly lscomp_compress_u16 of its frame alone, and its
+ * output range stays reserved until lscomp_server_release.  Two engines:
+ *  - lscomp_server_create_device: a device-resident request loop.  One
+ *    persistent kernel takes requests from a ring in host-pinned memory as
+ *    they are admitted, spreads their tiles over every workgroup (requests
+ *    that arrive while others run join at once) and writes each completion
+ *    into host-pinned memory; no kernel launch per request or batch.  A
+ *    kernel instance ends after `idle_us` without a request (or when its
+ *    tables are full) and a host thread starts the next one when requests
+ *    wait.  max_blocks bounds one request.
+ *  - lscomp_server_create: a host thread packs the requests admitted since its
+ *    last launch into one launch of one of `slots` plans (batched form), each
+ *    on its own stream, so batches overlap; max_items / max_blocks per batch.
+ * No counterpart in the reference. */
 typedef struct lscomp_request_u16 {
     lscomp_dim3 dims;
     lscomp_bins4 bins;
     float pooling_th;
-    const uint16_t *d_frame;   /* device input, alive until the request completes */
+    const uint16_t *d_frame;   /* device input, alive and unchanged until the request completes */
     uint64_t tag;              /* returned with the completion */
 } lscomp_request_u16;
 
 typedef struct lscomp_completion {
     uint64_t id;               /* from lscomp_server_submit_u16 */
     uint64_t tag;
-    int status;                /* LSCOMP_OK, or an error for this request */
+    int status;                /* LSCOMP_OK, or an error for this request (LSCOMP_ERR_INTERNAL: the
+                                  launch that ran it reported a look-back timeout) */
     uint32_t reserved;
     uint64_t stream_bytes;
-    const uint8_t *d_stream;   /* in the server's arena; valid until released */
+    const uint8_t *d_stream;   /* in the server's arena; valid until released (null on error) */
     double t_submit, t_admit, t_launch, t_done;   /* seconds (steady clock) */
 } lscomp_completion;
 
 typedef struct lscomp_server_s *lscomp_server;
 
-/* slots: batches in flight (streams); max_items / max_blocks: per batch. */
+typedef struct lscomp_server_stats_t {
+    uint64_t submitted, completed;
+    uint64_t launches;         /* kernel launches: batches (host engine) or loop instances (device engine) */
+    uint32_t device_loop;      /* 1 for the device engine */
+    uint32_t reserved;
+} lscomp_server_stats_t;
+
+/* Host engine: slots = batches in flight (streams); max_items / max_blocks per batch. */
 int lscomp_server_create(uint32_t slots, uint32_t max_items, uint64_t max_blocks, uint64_t budget_bytes,
                          lscomp_server *srv);
+/* Device engine: max_blocks per request, idle_us before an idle kernel instance ends. */
+int lscomp_server_create_device(uint64_t max_blocks, uint64_t budget_bytes, uint32_t idle_us, lscomp_server *srv);
+/* LSCOMP_ERR_TOO_LARGE for more than max_blocks blocks, LSCOMP_ERR_CAPACITY
+ * when the request's worst-case stream exceeds the whole budget. */
 int lscomp_server_submit_u16(lscomp_server srv, const lscomp_request_u16 *req, uint64_t *id);
 /* Non-blocking: copies up to max finished requests to out, sets *n. */
 int lscomp_server_poll(lscomp_server srv, lscomp_completion *out, uint32_t max, uint32_t *n);
 /* Frees the output buffer of a finished request (makes room for admission). */
 int lscomp_server_release(lscomp_server srv, uint64_t id);
-/* Blocks until every submitted request has finished. */
+/* Blocks until every submitted request has finished; returns the first error
+ * a completion reported since the last drain (or LSCOMP_OK). */
 int lscomp_server_drain(lscomp_server srv);
+int lscomp_server_stats(lscomp_server srv, lscomp_server_stats_t *out);
 /* Submits reqs[i] at time t0 + arrival_s[i] (busy-waiting on the calling
  * thread, t0 = call time), collects every completion into out[i] (by request
- * order) and releases each output as soon as it is copied out; returns when
- * all finished.  A trace replay for load tests (e.g. Poisson arrivals). */
+ * order); returns when all finished.  By default each output is released as
+ * soon as it is collected (d_stream is then null); with LSCOMP_REPLAY_HOLD
+ * the outputs stay reserved until the caller releases them.  A trace replay
+ * for load tests (e.g. Poisson arrivals). */
+enum { LSCOMP_REPLAY_HOLD = 1 };
 int lscomp_server_replay_u16(lscomp_server srv, const lscomp_request_u16 *reqs, const double *arrival_s,
-                             uint32_t n, lscomp_completion *out);
+                             uint32_t n, lscomp_completion *out, uint32_t flags);
 int lscomp_server_destroy(lscomp_server srv);
 
 /* ---- Error metrics ------------------------------------------------------

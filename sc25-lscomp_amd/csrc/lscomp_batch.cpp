@@ -229,6 +229,24 @@ int lscomp_plan_launch_graph(lscomp_plan p, hipStream_t stream)
 
 const uint64_t *lscomp_plan_device_sizes(lscomp_plan p) { return p ? p->d_sizes : nullptr; }
 
+}  // extern "C"
+
+namespace lsc {
+// Stream-ordered: copies the plan's error word into *h_err (pinned host
+// memory) and clears it, so a batch's completions can carry the launch's
+// status (served form).
+int plan_take_error_async(lscomp_plan p, uint32_t *h_err, hipStream_t stream)
+{
+    if (!p || !h_err) return LSCOMP_ERR_INVALID_ARG;
+    int st = hip_fail(hipMemcpyAsync(h_err, &p->ctrl->error, sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+                      "hipMemcpyAsync");
+    if (!st) st = hip_fail(launch_fill_u64((uint64_t *)&p->ctrl->error, 1, 0, stream), "fill");
+    return st;
+}
+}  // namespace lsc
+
+extern "C" {
+
 int lscomp_plan_sizes(lscomp_plan p, uint64_t *host_sizes, hipStream_t stream)
 {
     if (!p || !host_sizes) return LSCOMP_ERR_INVALID_ARG;

@@ -24,6 +24,9 @@ uint32_t align_class(const void *vol, uint32_t z);
 uint64_t num_blocks(lscomp_dim3 d);
 // Validates dims/bins/pooling and fills the geometry and decision parts of an Item.
 int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it, uint32_t tile);
+// Stream-ordered copy of a plan's error word (ERRF_*) into pinned host memory,
+// then cleared on the device (lscomp_batch.cpp).
+int plan_take_error_async(lscomp_plan p, uint32_t *h_err, hipStream_t stream);
 // Grid and scheduling of a launch of `total_tiles` tiles with a normalized shape.
 LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, const Shape &shape);
 

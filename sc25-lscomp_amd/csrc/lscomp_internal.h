@@ -84,6 +84,7 @@ struct Item {
     uint32_t pad1;
     FastDiv div_frame, div_nbz;
     Decide dec;
+    uint64_t seq;             // served form: the request's sequence number
 };
 
 // Launch shape of one direction: the stream never depends on it.
@@ -130,6 +131,67 @@ hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t value, hipStream_t 
 constexpr uint32_t kTileU32 = 256;
 hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,
                       hipStream_t stream);
+// ---------------------------------------------------------------------------
+// Served form: a device-resident request loop (serve_kernel, lscomp_server.cpp).
+// The host writes prepared items into a ring in host-pinned coherent memory
+// and publishes a write index; one workgroup of the persistent kernel (the
+// dispatcher) copies new items into a device item table, gives each its tile
+// range and publishes the tiles through tagged tile-map words; every other
+// workgroup compresses published tiles and the last tile of a request writes
+// its completion record into host-pinned memory.  A kernel instance closes
+// after an idle period, on the host's stop word or when its item table or
+// tile map is full; the host relaunches it when requests wait.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSrvRing = 4096;            // ring slots (requests queued or running)
+constexpr uint32_t kSrvMaxReq = 1u << 15;      // requests per kernel instance (device item table)
+constexpr uint32_t kSrvMaxTiles = 1u << 22;    // tiles per kernel instance (tile map, look-back words)
+constexpr uint32_t kSrvTile = 512;             // blocks per tile of the served kernel
+
+struct alignas(128) ItemSlot {
+    Item it;
+};
+// One ring slot, written by the host before it publishes `wr`.  The item is
+// complete except first_tile and result, which the dispatcher sets.
+struct alignas(128) SrvReq {
+    Item item;
+};
+// Completion record of a ring slot, written by the device: bytes and status,
+// then seq = the request's sequence number + 1 (system-scope release).
+struct alignas(32) SrvComp {
+    uint64_t seq;
+    uint64_t bytes;
+    uint32_t status;      // 0, or SRV_CAPACITY / SRV_TIMEOUT
+    uint32_t pad[3];
+};
+enum : uint32_t { SRV_CAPACITY = 1u, SRV_TIMEOUT = 2u };
+// Host-pinned control words, one line each.
+struct SrvHost {
+    uint64_t wr;          // host: requests published (sequence numbers below wr are in the ring)
+    uint64_t pad0[15];
+    uint64_t rd;          // device: requests the dispatchers have taken (all instances)
+    uint64_t pad1[15];
+    uint32_t stop;        // host: the running instance closes
+    uint32_t pad2[31];
+};
+// Device state of one kernel instance, cleared by the host before each launch.
+struct SrvDev {
+    uint64_t close;       // (tile limit << 1) | 1: no tile at or past the limit will be published
+    uint64_t pad0[15];
+};
+struct SrvArgs {
+    const SrvReq *ring;   // host-pinned
+    SrvComp *comp;        // host-pinned
+    SrvHost *host;        // host-pinned
+    SrvDev *dev;
+    ItemSlot *items;      // kSrvMaxReq
+    uint64_t *map;        // kSrvMaxTiles: epoch | item + 1 | tile within the item
+    uint32_t *done;       // kSrvMaxReq: tiles stored per item
+    uint64_t *sizes;      // kSrvMaxReq: stream bytes per item
+    uint64_t idle_ticks;  // s_memrealtime ticks (100 MHz) without a new request before an instance closes
+};
+hipError_t launch_serve(const SrvArgs &a, uint32_t grid, Sched sc, Ctrl *ctrl, uint64_t *status, hipStream_t stream);
+int workgroups_per_cu_serve();
+
 int workgroups_per_cu(bool compress, bool table, int tile);   // occupancy limit
 int workgroups_per_cu_u32(bool compress);
 int num_cus();

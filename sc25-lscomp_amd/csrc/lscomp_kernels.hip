@@ -1463,6 +1463,339 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
 }
 
 // ---------------------------------------------------------------------------
+// Served form: the device-resident request loop (SrvArgs, lscomp_internal.h).
+// The workgroup that takes ticket 0 is the dispatcher; every other one runs
+// the compression pipeline above on published tiles, from counter class
+// (ticket - 1) mod nc.
+// ---------------------------------------------------------------------------
+template <int TILE>
+struct SrvShared {
+    TileShared<TILE> t;                               // the pipeline's LDS (the dispatcher uses t.pay as scratch)
+    alignas(16) uint32_t itm[4][sizeof(Item) / 4];    // item of each tile in flight, by header buffer
+    uint32_t nst[2], nr[2], ntl[2];                   // state of the next tile, by iteration parity
+    uint64_t wr, now;
+    uint32_t stop, total;
+};
+
+// Tile-map word of a published tile: epoch (26 bits) | item + 1 (16 bits) |
+// tile within the item (22 bits).  A cleared word (item field 0) never passes.
+__device__ __forceinline__ uint64_t map_word(uint32_t epoch, uint32_t r, uint32_t tl)
+{
+    return ((uint64_t)epoch << 38) | ((uint64_t)(r + 1) << 22) | tl;
+}
+
+// The dispatcher: takes new requests from the host ring in order, copies each
+// item into the item table (setting its first tile and its size slot), clears
+// its tile count, makes all of that visible (barrier, agent-scope release),
+// then publishes every new tile's map word.  Closes the instance after
+// idle_ticks without a request, on the host's stop word, or when the item
+// table or the tile map is full (the requests left over stay in the ring for
+// the next instance).  Called by every thread of its workgroup.
+template <int TILE>
+__device__ __forceinline__ void serve_dispatch(const SrvArgs &a, uint32_t epoch, SrvShared<TILE> &sh)
+{
+    constexpr uint32_t kW = sizeof(Item) / 4;
+    const uint32_t tid = threadIdx.x;
+    uint32_t *ex = sh.t.pay;   // first tile of each new request, relative to the instance's tile count
+    if (tid == 0) {
+        sh.wr = __hip_atomic_load(&a.host->rd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        sh.now = __builtin_amdgcn_s_memrealtime();
+    }
+    __syncthreads();
+    uint64_t rd = sh.wr, last = sh.now;   // next request to take; time of the last one taken
+    uint32_t nreq = 0, ntiles = 0;        // items and tiles published by this instance
+    __syncthreads();
+    for (;;) {
+        if (tid == 0) {
+            sh.wr = __hip_atomic_load(&a.host->wr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            sh.stop = __hip_atomic_load(&a.host->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            sh.now = __builtin_amdgcn_s_memrealtime();
+        }
+        __syncthreads();
+        const uint64_t wr = sh.wr, now = sh.now;
+        const uint32_t stop = sh.stop;
+        uint64_t avail = wr > rd ? wr - rd : 0;
+        if (avail > (uint64_t)TILE) avail = TILE;
+        if (avail > (uint64_t)(kSrvMaxReq - nreq)) avail = kSrvMaxReq - nreq;
+        const uint32_t take = (uint32_t)avail;
+        if (take == 0) {
+            if (stop || now - last > a.idle_ticks || nreq == kSrvMaxReq) break;
+            __syncthreads();   // everyone has read sh.* before thread 0 rewrites it
+            __builtin_amdgcn_s_sleep(8);
+            continue;
+        }
+        // tile counts of the new requests, one per thread; a request whose
+        // tiles would pass the end of the tile map waits for the next instance
+        uint32_t nt = 0;
+        if (tid < take)
+            nt = __hip_atomic_load(&a.ring[(rd + tid) % kSrvRing].item.ntiles, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+        uint32_t agg;
+        const uint32_t off = tile_scan<TILE>(nt, tid, sh.t.wsum[0], &agg);
+        const bool fit = tid < take && (uint64_t)ntiles + off + nt <= kSrvMaxTiles;
+        const uint32_t nfit = (uint32_t)__syncthreads_count(fit);   // a prefix of the new requests
+        if (nfit == 0) break;
+        if (fit) ex[tid] = off;
+        if (tid + 1 == nfit) sh.total = off + nt;
+        // copy the items from the ring
+        for (uint32_t k = tid; k < nfit * kW; k += TILE) {
+            const uint32_t j = k / kW, w = k - j * kW;
+            const uint32_t *src = (const uint32_t *)&a.ring[(rd + j) % kSrvRing].item;
+            ((uint32_t *)&a.items[nreq + j].it)[w] =
+                __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __syncthreads();
+        const uint32_t total = sh.total;
+        if (tid < nfit) {
+            Item &it = a.items[nreq + tid].it;
+            it.first_tile = ntiles + ex[tid];
+            it.result = a.sizes + nreq + tid;
+            a.done[nreq + tid] = 0;
+        }
+        // items and tile counts reach memory before any map word
+        __syncthreads();
+        if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        for (uint32_t t = tid; t < total; t += TILE) {
+            uint32_t lo = 0, hi = nfit - 1;   // the request j with ex[j] <= t < ex[j + 1]
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (ex[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            st_relaxed(a.map + ntiles + t, map_word(epoch, nreq + lo, t - ex[lo]));
+        }
+        nreq += nfit;
+        ntiles += total;
+        rd += nfit;
+        last = now;
+        if (tid == 0) __hip_atomic_store(&a.host->rd, rd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();   // ex[] and sh.* are rewritten next round
+    }
+    // no tile at or past ntiles will be published by this instance
+    if (tid == 0) st_relaxed(&a.dev->close, ((uint64_t)ntiles << 1) | 1u);
+}
+
+// A consumer's tile of item r is stored (every thread's stores issued): make
+// them visible at system scope, count the tile, and if it was the item's last
+// one write the completion record (bytes, status, then seq + 1).  Called by
+// every thread; one barrier.
+__device__ __forceinline__ void serve_tile_done(const SrvArgs &a, Ctrl *ctrl, uint32_t r, const Item &it)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        const uint32_t old = __hip_atomic_fetch_add(a.done + r, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == it.ntiles) {
+            // every tile is stored; the last one (in block order) wrote the size
+            const uint64_t bytes = ld_relaxed(a.sizes + r);
+            uint32_t st = bytes > it.s_len ? SRV_CAPACITY : 0u;
+            if (__hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERRF_TIMEOUT)
+                st |= SRV_TIMEOUT;
+            SrvComp *c = a.comp + it.seq % kSrvRing;
+            __hip_atomic_store(&c->bytes, bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&c->status, st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&c->seq, it.seq + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
+// A consumer: the compression pipeline of compress_tiles (same stages, same
+// LDS ring) on published tiles.  Before each iteration thread 0 reads the map
+// word of the reserved tile n with an agent-scope acquire (the item and the
+// frame are read after it; the acquire also drops stale cache lines of a
+// frame buffer the host reused).  When n is not yet published the iteration
+// finishes pending tiles instead, so no tile waits for one that a waiting
+// workgroup holds; with nothing pending the workgroup sleeps, and it leaves
+// once the dispatcher has closed the instance below n.
+template <int TILE>
+__device__ __forceinline__ void serve_consume(const SrvArgs &a, Ctrl *ctrl, uint64_t *status, Sched sc,
+                                              uint32_t cls, uint32_t epoch, SrvShared<TILE> &ss)
+{
+    using C = CodecU16T<true>;
+    constexpr uint32_t kRing = (uint32_t)TILE * C::kMaxWords;
+    constexpr uint32_t kW = sizeof(Item) / 4;
+    TileShared<TILE> &sh = ss.t;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t win = sc.win;
+    const TileQ q = {cls, sc.nc};
+    if (tid == 0) sh.next = q.take(ctrl);
+    __syncthreads();
+    const auto item_in = [&](uint32_t buf) -> const Item & { return *(const Item *)ss.itm[buf]; };
+
+    uint32_t np = 0, hb = 0;
+    PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
+    uint32_t n = sh.next, par = 0;
+    uint32_t late = 0;   // thread 0: rounds waited for n after the instance closed above it
+    for (;; par ^= 1u) {
+        // 0. is n published? (1: yes, 2: never in this instance, 0: not yet)
+        if (tid == 0) {
+            uint32_t st = 0;
+            if (n < kSrvMaxTiles) {
+                const uint64_t w = __hip_atomic_load(a.map + n, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t r1 = (uint32_t)(w >> 22) & 0xffffu;
+                if ((uint32_t)(w >> 38) == epoch && r1 != 0) {
+                    st = 1;
+                    ss.nr[par] = r1 - 1;
+                    ss.ntl[par] = (uint32_t)w & 0x3fffffu;
+                }
+            }
+            if (st == 0) {
+                const uint64_t c = ld_relaxed(&a.dev->close);
+                if ((c & 1u) && (uint64_t)n >= (c >> 1)) st = 2;
+                // closed above n, yet n's word never shows: give up (bounded wait)
+                if ((c & 1u) && st == 0 && ++late > kSpinLimit) {
+                    __hip_atomic_fetch_or(&ctrl->error, ERRF_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    st = 2;
+                }
+            }
+            ss.nst[par] = st;
+        }
+        __syncthreads();
+        const uint32_t st = ss.nst[par];
+        const bool have_n = st == 1;
+        if (!have_n && np == 0) {
+            if (st == 2) break;
+            __builtin_amdgcn_s_sleep(4);
+            continue;
+        }
+        const bool resolve = np == 2 || (!have_n && np != 0);
+        const uint32_t n_item = have_n ? ss.nr[par] : 0u, n_tl = have_n ? ss.ntl[par] : 0u;
+        if (have_n) {   // the item of n into its buffer (last used four tiles ago)
+            if (tid < kW) ss.itm[hb][tid] = ((const uint32_t *)&a.items[n_item].it)[tid];
+            __syncthreads();
+        }
+
+        // 1. loads of n, probe of p0, next id
+        uint32_t D[C::kRegs];
+        bool nvalid = false;
+        if (have_n) {
+            const Item &nit = item_in(hb);
+            const uint32_t b = n_tl * (uint32_t)TILE + tid;
+            nvalid = b < nit.nblocks;
+            if (nvalid) {
+                uint32_t bx, by, bz;
+                block_coords(nit, b, bx, by, bz);
+                C::load(nit, bx, by, bz, D);
+            }
+        }
+        const bool p_lb = resolve && p0.tl != 0;
+        uint64_t w0[lb_per(TILE)];
+        if (p_lb) lb_probe<TILE>(status, (int64_t)p0.tile - 1, item_in(p0.hb).first_tile, epoch, win, w0);
+        uint32_t next_id = n;
+        if (tid == 0 && have_n) next_id = q.take(ctrl);
+
+        // 2. header of n; scan and the first half of p0's evaluation share a barrier
+        uint32_t n_hdr = 0, n_words = 0;
+        if (have_n) {
+            if (nvalid) {
+                n_hdr = C::header(D, item_in(hb), ctrl);
+                n_words = header_payload_bytes(n_hdr) >> 2;
+            }
+            ((uint8_t *)sh.hdr[hb])[tid] = (uint8_t)n_hdr;
+        }
+        const uint32_t incl = wave_incl_scan(n_words, lane);
+        if (lane == 63) sh.wsum[par][wave] = incl;
+        if (p_lb) lb_slices<TILE>(w0, epoch, win, sh.lbf, par);
+        if (tid == 0) sh.next2[par] = next_id;
+        __syncthreads();
+        const uint32_t n_next = sh.next2[par];
+        uint32_t before = 0, n_agg = 0;
+#pragma unroll
+        for (int v = 0; v < TILE / 64; v++) {
+            const uint32_t x = sh.wsum[par][v];
+            before += (v < (int)wave) ? x : 0u;
+            n_agg += x;
+        }
+        const uint32_t n_off = before + incl - n_words;
+        if (have_n && tid == 0) st_relaxed(status + n, lb_word(epoch, n_tl == 0 ? LB_INCL : LB_AGG, n_agg));
+
+        // 3. pack n behind the pending tiles if it fits in the staging ring
+        uint32_t n_start = have_n ? ring_place(n_agg, np, p0, p1, kRing) : ~0u;
+        if (n_start != ~0u && n_words) C::encode(D, n_hdr, item_in(hb), &sh.pay[n_start + n_off]);
+
+        // 4. finish p0
+        if (resolve) {
+            const Item &pit = item_in(p0.hb);
+            uint64_t excl = 0;
+            if (p_lb) {
+                const uint32_t r = lb_reduce<TILE>(sh.lbf, par, win, excl);
+                if (r != LB_DONE) {
+                    const int64_t base = (int64_t)p0.tile - 1 - (r == LB_NEXT ? (int64_t)win : 0);
+                    excl = lookback_from<TILE>(status, base, pit.first_tile, epoch, win, sh.lb, ctrl, excl,
+                                               r == LB_RETRY ? 1u : 0u, 0u);
+                }
+                if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
+            }
+            store_tile<TILE>(pit, p0.tl, p0.agg, sh.hdr[p0.hb], sh.pay + p0.start, excl, tid);
+            serve_tile_done(a, ctrl, p0.item, pit);
+            p0 = p1;
+            np--;
+        }
+        if (have_n && n_start == ~0u) {
+            __syncthreads();
+            for (;;) {
+                n_start = ring_place(n_agg, np, p0, p1, kRing);
+                if (n_start != ~0u) break;
+                const Item &pit = item_in(p0.hb);
+                uint64_t excl = 0;
+                if (p0.tl != 0) {
+                    excl = lookback_tile<TILE>(status, p0.tile, pit.first_tile, epoch, win, sh.lb, ctrl);
+                    if (tid == 0) st_relaxed(status + p0.tile, lb_word(epoch, LB_INCL, excl + p0.agg));
+                }
+                store_tile<TILE>(pit, p0.tl, p0.agg, sh.hdr[p0.hb], sh.pay + p0.start, excl, tid);
+                serve_tile_done(a, ctrl, p0.item, pit);
+                p0 = p1;
+                np--;
+                __syncthreads();
+            }
+            if (n_words) C::encode(D, n_hdr, item_in(hb), &sh.pay[n_start + n_off]);
+            __syncthreads();
+        }
+        if (have_n) {
+            const PendTile t = {n, n_tl, n_item, n_agg, n_start, hb};
+            if (np == 0) p0 = t; else p1 = t;
+            np++;
+            hb = (hb + 1u) & 3u;
+            n = n_next;
+        }
+    }
+}
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void serve_kernel(
+    const SrvArgs a, Ctrl *ctrl, uint64_t *status, Sched sc)
+{
+    __shared__ SrvShared<TILE> sh;
+    if (threadIdx.x == 0) {
+        sh.t.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh.t.cls = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t epoch = sh.t.epoch;
+    const uint32_t ticket = __builtin_amdgcn_readfirstlane(sh.t.cls);
+    __syncthreads();
+    if (ticket == 0) serve_dispatch<TILE>(a, epoch, sh);
+    else serve_consume<TILE>(a, ctrl, status, sc, (ticket - 1) % sc.nc, epoch, sh);
+    finish_launch(ctrl, epoch);
+}
+
+hipError_t launch_serve(const SrvArgs &a, uint32_t grid, Sched sc, Ctrl *ctrl, uint64_t *status, hipStream_t stream)
+{
+    if (grid < 2 || sc.nc < 1 || sc.nc > grid - 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((serve_kernel<(int)kSrvTile>), dim3(grid), dim3(kSrvTile), 0, stream, a, ctrl, status, sc);
+    return hipGetLastError();
+}
+
+int workgroups_per_cu_serve()
+{
+    int per_cu = 0;
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel<(int)kSrvTile>, (int)kSrvTile, 0);
+    return (e == hipSuccess && per_cu > 0) ? per_cu : 1;
+}
+
+// ---------------------------------------------------------------------------
 // uint32 volumes (replace cuLSZ_{compression,decompression}_kernel_uint32_bsize64,
 // lsCOMP/src/cuLSZ_kernel.cu:4-824): the uint16 compression and pipelined
 // decompression with the uint32 codec (CodecU32: one block of 64 uint32 values

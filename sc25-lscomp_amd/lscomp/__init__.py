@@ -78,8 +78,17 @@ EXPORTS = (
     "lscomp_max_compressed_bytes_u32", "lscomp_compress_u32", "lscomp_compress_u32_async",
     "lscomp_decompress_u32", "lscomp_decompress_u32_async",
     "lscomp_container_bytes", "lscomp_container_pack", "lscomp_container_parse",
-    "lscomp_set_launch_shape_ex", "lscomp_get_launch_shape_ex",
+    "lscomp_set_launch_shape_ex", "lscomp_get_launch_shape_ex", "lscomp_server_create_device",
+    "lscomp_server_stats",
 )
+
+REPLAY_HOLD = 1
+
+
+class ServerStats(ctypes.Structure):
+    """lscomp_server_stats_t."""
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("device_loop", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class LaunchShape(ctypes.Structure):
@@ -189,6 +198,10 @@ def _load():
     lib.lscomp_server_create.restype = ctypes.c_int
     lib.lscomp_server_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
                                          ctypes.POINTER(vp)]
+    lib.lscomp_server_create_device.restype = ctypes.c_int
+    lib.lscomp_server_create_device.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(vp)]
+    lib.lscomp_server_stats.restype = ctypes.c_int
+    lib.lscomp_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     lib.lscomp_server_submit_u16.restype = ctypes.c_int
     lib.lscomp_server_submit_u16.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
     lib.lscomp_server_poll.restype = ctypes.c_int
@@ -200,7 +213,7 @@ def _load():
     lib.lscomp_server_drain.argtypes = [vp]
     lib.lscomp_server_replay_u16.restype = ctypes.c_int
     lib.lscomp_server_replay_u16.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double),
-                                             ctypes.c_uint32, ctypes.POINTER(Completion)]
+                                             ctypes.c_uint32, ctypes.POINTER(Completion), ctypes.c_uint32]
     lib.lscomp_server_destroy.restype = ctypes.c_int
     lib.lscomp_server_destroy.argtypes = [vp]
     lib.lscomp_error_stats_u16.restype = ctypes.c_int
@@ -619,14 +632,34 @@ class Plan:
 
 
 class Server:
-    """Served form: continuous batching of frame-compression requests (see include/lscomp.h)."""
+    """Served form: continuous batching of frame-compression requests (see include/lscomp.h).
 
-    def __init__(self, slots: int, max_items: int, max_blocks: int, budget_bytes: int):
+    ``Server(slots, max_items, max_blocks, budget_bytes)`` is the host engine
+    (host-scheduled batch launches); ``Server.device(max_blocks, budget_bytes,
+    idle_us)`` the device-resident request loop."""
+
+    def __init__(self, slots: int = 0, max_items: int = 0, max_blocks: int = 1 << 22, budget_bytes: int = 1 << 30,
+                 device: bool = False, idle_us: int = 2000):
         h = ctypes.c_void_p()
-        st = _lib.lscomp_server_create(slots, max_items, max_blocks, budget_bytes, ctypes.byref(h))
+        if device:
+            st = _lib.lscomp_server_create_device(max_blocks, budget_bytes, idle_us, ctypes.byref(h))
+        else:
+            st = _lib.lscomp_server_create(slots, max_items, max_blocks, budget_bytes, ctypes.byref(h))
         if st != OK:
-            raise LscompError(st, "lscomp_server_create")
+            raise LscompError(st, "lscomp_server_create_device" if device else "lscomp_server_create")
         self.h = h
+
+    @classmethod
+    def device(cls, max_blocks: int, budget_bytes: int, idle_us: int = 2000) -> "Server":
+        return cls(max_blocks=max_blocks, budget_bytes=budget_bytes, device=True, idle_us=idle_us)
+
+    def stats(self) -> dict:
+        out = ServerStats()
+        st = _lib.lscomp_server_stats(self.h, ctypes.byref(out))
+        if st != OK:
+            raise LscompError(st, "lscomp_server_stats")
+        return {"submitted": out.submitted, "completed": out.completed, "launches": out.launches,
+                "device_loop": bool(out.device_loop)}
 
     @staticmethod
     def request(frame, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, tag: int = 0) -> Request:
@@ -656,13 +689,14 @@ class Server:
     def drain(self) -> int:
         return int(_lib.lscomp_server_drain(self.h))
 
-    def replay(self, reqs, arrivals):
-        """Submit reqs at the given arrival times (s); returns Completion per request, in order."""
+    def replay(self, reqs, arrivals, hold: bool = False):
+        """Submit reqs at the given arrival times (s); returns Completion per request, in order.
+        With ``hold`` the outputs stay reserved (release them with ``release``)."""
         n = len(reqs)
         arr = (Request * n)(*reqs)
         times = (ctypes.c_double * n)(*[float(t) for t in arrivals])
         out = (Completion * n)()
-        st = _lib.lscomp_server_replay_u16(self.h, arr, times, n, out)
+        st = _lib.lscomp_server_replay_u16(self.h, arr, times, n, out, REPLAY_HOLD if hold else 0)
         if st != OK:
             raise LscompError(st, "lscomp_server_replay_u16")
         return [out[i] for i in range(n)]
