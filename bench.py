@@ -1,13 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 lsCOMP uint16 codec on one MI355X.
 
-Workload (default): the CSSI-shaped volume 600 x 1813 x 1558 uint16 (3.39 GB),
-Poisson(0.3) background + 50 Gaussian peaks per frame + gap stripes, seed 2,
-generated on the host and resident in HBM before timing.  A step is one
-compression of the whole volume (lossless: bins 1 1 1 1, pooling 1); the same
-K steps are then timed for decompression and for the lossy setting
-(bins 3 5 10 15, pooling 0.5).  `value` is the lossless compression
-throughput in GB/s (1e9 bytes of uncompressed input per second).
+Workload (the headline `value`): the CSSI-shaped volume 600 x 1813 x 1558
+uint16 (3.39 GB), Poisson(0.3) background + 50 Gaussian peaks per frame + gap
+stripes, seed 2, generated on the host and resident in HBM before timing.  A
+step is one compression of the whole volume (lossless: bins 1 1 1 1,
+pooling 1); `value` is its throughput in GB/s (1e9 bytes of uncompressed input
+per second).  The same K steps are timed for decompression and for the lossy
+setting (bins 3 5 10 15, pooling 0.5) and reported in `modes`, with the
+XPCS-shaped 512 x 1024 x 1024 volume (lossless and bins 3/5/10 x4, pooling 0.5),
+the small 16 x 512 x 512 volume (checked byte for byte against the oracle) and
+the uint32 path.  `forms` holds the batched form (512 uneven frames, one
+autotuned plan replayed from a hipGraph, against 512 plain launches) and the
+served form (512^2..2048^2 CSSI frames, half lossless and half lossy, under a
+64 GB budget: the device-resident loop at the offered 20k frames/s and at
+saturation, beside the host-scheduled engine and plain per-frame launches).
 
 Prints ONE JSON line.  `--gpus` above 1 is refused: this build runs on one GPU.
 """
@@ -17,6 +24,7 @@ import argparse
 import hashlib
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -26,6 +34,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 CSSI_DIMS = (600, 1813, 1558)
+XPCS_DIMS = (512, 1024, 1024)
+SMALL_DIMS = (16, 512, 512)
 
 
 def log(msg):
@@ -63,33 +73,128 @@ def profiled_traffic(dims):
     return prof.get("traffic_bytes")
 
 
-def cpu_baseline(vol, sample_frames, bins, pooling, single_frames=8):
-    """Oracle (C, OpenMP two-pass) on the first `sample_frames` frames, all host
-    threads; and the serial oracle (one thread) on the first `single_frames`."""
+def time_codec(torch, lscomp, d_in, dims, bins, pool, d_out, d_dec, d_size, args, in_bytes, check_lossless):
+    """Compress and decompress timing of one setting (K steps each after W warm-ups)."""
+    comp = lambda: lscomp.compress_async(d_in, dims, bins, pool, d_out, d_size)
+    for _ in range(args.warmup):
+        comp()
+    torch.cuda.synchronize()
+    c_wall, c_ev, _ = timed_loop(torch, comp, args.steps)
+    s_bytes = int(d_size.item())
+    dec = lambda: lscomp.decompress_async(d_out, s_bytes, dims, bins, d_dec)
+    for _ in range(args.warmup):
+        dec()
+    torch.cuda.synchronize()
+    d_wall, d_ev, _ = timed_loop(torch, dec, args.steps)
+    ok = lscomp.stream_status() == lscomp.OK
+    res = {
+        "compress_GBps": in_bytes / (c_wall * 1e-3) / 1e9,
+        "decompress_GBps": in_bytes / (d_wall * 1e-3) / 1e9,
+        "compress_ms": c_wall, "decompress_ms": d_wall,
+        "compress_kernel_ms": c_ev, "decompress_kernel_ms": d_ev,
+        "stream_bytes": s_bytes, "ratio": in_bytes / s_bytes,
+        "compress_hbm_frac": (in_bytes + s_bytes) / (c_ev * 1e-3) / HBM_PEAK,
+        "decompress_hbm_frac": (in_bytes + s_bytes) / (d_ev * 1e-3) / HBM_PEAK,
+    }
+    if check_lossless:
+        ok = ok and bool(torch.equal(d_dec, d_in))
+    else:
+        diff = ((d_in.to(torch.int32) & 0xFFFF) - (d_dec.to(torch.int32) & 0xFFFF)).abs()
+        res["max_abs_err"] = int(diff.max().item())
+        del diff
+    res["verified"] = ok
+    return res
+
+
+def bench_xpcs(torch, lscomp, datagen, np, args):
+    """XPCS-shaped 512 x 1024 x 1024: lossless and bins 3/5/10 x4 with pooling 0.5."""
+    dims = XPCS_DIMS
+    vol = datagen.volume("xpcs", 3, dims)
+    d_in = torch.from_numpy(vol.view(np.int16)).cuda()
+    in_bytes = 2 * vol.size
+    d_out = torch.empty(lscomp.max_compressed_bytes(dims), dtype=torch.uint8, device="cuda")
+    d_dec = torch.empty(dims, dtype=torch.int16, device="cuda")
+    d_size = torch.zeros(1, dtype=torch.int64, device="cuda")
+    lscomp.reserve(dims)
+    res = {}
+    for name, bins, pool in (("xpcs_lossless", (1, 1, 1, 1), 1.0), ("xpcs_b3x4_p0.5", (3, 3, 3, 3), 0.5),
+                             ("xpcs_b5x4_p0.5", (5, 5, 5, 5), 0.5), ("xpcs_b10x4_p0.5", (10, 10, 10, 10), 0.5)):
+        res[name] = time_codec(torch, lscomp, d_in, dims, bins, pool, d_out, d_dec, d_size, args, in_bytes,
+                               bins == (1, 1, 1, 1))
+        log(f"{name}: {json.dumps(res[name])}")
+    del d_in, d_out, d_dec
+    return res
+
+
+def bench_small(torch, lscomp, datagen, np, args):
+    """The 16 x 512 x 512 sparse volume (Poisson 0.05 + 0.1 % hot pixels), lossless,
+    byte-equal to the oracle's stream and round-tripped exactly."""
+    import oracle
+    dims = SMALL_DIMS
+    vol = datagen.volume("sparse", 1, dims)
+    d_in = torch.from_numpy(vol.view(np.int16)).cuda()
+    in_bytes = 2 * vol.size
+    d_out = torch.empty(lscomp.max_compressed_bytes(dims), dtype=torch.uint8, device="cuda")
+    d_dec = torch.empty(dims, dtype=torch.int16, device="cuda")
+    d_size = torch.zeros(1, dtype=torch.int64, device="cuda")
+    r = time_codec(torch, lscomp, d_in, dims, (1, 1, 1, 1), 1.0, d_out, d_dec, d_size, args, in_bytes, True)
+    _, ref, refn = oracle.get().compress(vol, (1, 1, 1, 1), 1.0)
+    r["oracle_identical"] = bool(refn == r["stream_bytes"] and np.array_equal(d_out[:refn].cpu().numpy(), ref))
+    r["verified"] = r["verified"] and r["oracle_identical"]
+    log(f"small_lossless: {json.dumps(r)}")
+    return {"small_16x512x512_lossless": r}
+
+
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def cpu_baseline(vol, bins, pooling, frames_all=48, frames_one=8, reps=5):
+    """The oracle (C; OpenMP two-pass variant on all threads, serial variant on
+    one) on the first frames of the volume: one warm-up, then the median of
+    `reps` passes of the same in-memory buffer."""
     import numpy as np
     import oracle
     orc = oracle.get()
 
-    def rate(sub, omp, limit_s, max_reps):
+    def median_pass(sub, omp):
         orc.compress(sub, bins, pooling, omp=omp)           # warm-up
-        reps, t0 = 0, time.perf_counter()
-        while True:
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
             orc.compress(sub, bins, pooling, omp=omp)
-            reps += 1
-            if time.perf_counter() - t0 > limit_s or reps >= max_reps:
-                break
-        return (time.perf_counter() - t0) / reps
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times)
 
-    sub = np.ascontiguousarray(vol[:sample_frames])
-    dt = rate(sub, True, 5.0, 5)
-    one = np.ascontiguousarray(vol[:min(single_frames, sample_frames)])
-    dt1 = rate(one, False, 3.0, 3)
-    return sub.nbytes / dt / 1e9, orc.max_threads(), sub.nbytes, dt, one.nbytes / dt1 / 1e9, one.shape[0]
+    sub = np.ascontiguousarray(vol[:frames_all])
+    dt = median_pass(sub, True)
+    one = np.ascontiguousarray(vol[:frames_one])
+    dt1 = median_pass(one, False)
+    threads = orc.max_threads()
+    return {"value": sub.nbytes / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"lossless compression of the first {frames_all} frames ({sub.nbytes / 1e9:.3f} GB) of the "
+                      f"same volume, OpenMP two-pass oracle on {threads} threads, 1 warm-up + median of {reps} "
+                      f"passes ({dt:.3f} s)",
+            "value_1_thread": one.nbytes / dt1 / 1e9,
+            "sample_1_thread": f"serial oracle, first {frames_one} frames ({one.nbytes / 1e9:.3f} GB), "
+                               f"1 warm-up + median of {reps} ({dt1:.3f} s)",
+            "cpu_model": cpu_info(), "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def bench_batched(torch, lscomp, datagen, np, steps):
     """512 square frames (sides drawn from 256..2048, XPCS-style, seed 4): one persistent
-    plan launch (replayed from a hipGraph) vs 512 plain per-frame launches; identical bytes."""
+    plan launch (shape autotuned on the batch, replayed from a hipGraph) vs 512 plain
+    per-frame launches; identical bytes."""
     rng = np.random.default_rng(4)
     sides = rng.choice([256, 512, 768, 1024, 1536, 2048], size=512)
     frames = [torch.from_numpy(datagen.volume("xpcs", 4, (1, int(s), int(s)), first_frame=i).view(np.int16)).cuda()
@@ -114,6 +219,12 @@ def bench_batched(torch, lscomp, datagen, np, steps):
     for fn in (plain, batch):
         fn()
     torch.cuda.synchronize()
+    _, default_ms, _ = timed_loop(torch, batch, steps)
+    default_shape = plan.get_shape()
+    table = plan.autotune(reps=3)
+    tuned_shape = plan.get_shape()
+    batch()
+    torch.cuda.synchronize()
     _, plain_ms, _ = timed_loop(torch, plain, steps)
     _, batch_ms, _ = timed_loop(torch, batch, steps)
     bsz = plan.sizes()
@@ -125,12 +236,17 @@ def bench_batched(torch, lscomp, datagen, np, steps):
             "plain_launches_ms": plain_ms, "plain_frames_per_s": len(frames) / (plain_ms * 1e-3),
             "batched_graph_ms": batch_ms, "batched_frames_per_s": len(frames) / (batch_ms * 1e-3),
             "batched_GBps": in_bytes / (batch_ms * 1e-3) / 1e9, "speedup": plain_ms / batch_ms,
+            "tuned_shape": tuned_shape, "default_shape": default_shape, "default_shape_ms": default_ms,
+            "tuner_shapes_measured": len(table),
             "identical_streams": bool(same), "verified": bool(ok)}
 
 
-def bench_served(torch, lscomp, datagen, np, seconds, rate):
-    """Poisson arrivals at `rate` frames/s for `seconds`: frames 512^2..2048^2 CSSI-style,
-    half lossless, half bins 3 5 10 15 / pooling 0.5, admitted by a 64 GB HBM budget."""
+def bench_served(torch, lscomp, datagen, np, seconds, rate, n_sat):
+    """512^2..2048^2 CSSI frames, half lossless and half bins 3 5 10 15 / pooling 0.5, 64 GB
+    budget.  The device-resident loop at the offered rate (Poisson arrivals) and at
+    saturation (every request offered at once), the host-scheduled engine at saturation,
+    and the same requests as plain per-frame launches.  Served streams are compared with
+    the plain ones."""
     rng = np.random.default_rng(5)
     pool_sides = [512, 768, 1024, 1536, 2048]
     pool = []
@@ -138,26 +254,96 @@ def bench_served(torch, lscomp, datagen, np, seconds, rate):
         s = pool_sides[i % len(pool_sides)]
         v = datagen.volume("cssi", 5, (1, s, s), first_frame=i)
         pool.append((torch.from_numpy(v.view(np.int16)).cuda(), (1, s, s)))
-    n = int(seconds * rate)
-    pick = rng.integers(0, len(pool), n)
-    reqs = []
-    for i, k in enumerate(pick):
-        f, d = pool[k]
-        b, p = ((1, 1, 1, 1), 1.0) if i % 2 == 0 else ((3, 5, 10, 15), 0.5)
-        reqs.append(lscomp.Server.request(f, d, b, p, tag=i))
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    srv = lscomp.Server(slots=4, max_items=256, max_blocks=1 << 22, budget_bytes=64 << 30)
-    t0 = time.perf_counter()
-    out = srv.replay(reqs, arrivals)
-    wall = time.perf_counter() - t0
-    ok = srv.drain() == lscomp.OK and all(c.status == lscomp.OK for c in out)
+    modes = [((1, 1, 1, 1), 1.0), ((3, 5, 10, 15), 0.5)]
+    budget = 64 << 30
+
+    def trace(n, seed):
+        r = np.random.default_rng(seed)
+        return [(int(k), i % 2) for i, k in enumerate(r.integers(0, len(pool), n))]
+
+    def requests(tr):
+        return [lscomp.Server.request(pool[k][0], pool[k][1], *modes[m], tag=i) for i, (k, m) in enumerate(tr)]
+
+    def frame_bytes(tr):
+        return sum(2 * pool[k][1][1] * pool[k][1][2] for k, _ in tr)
+
+    # plain per-frame launches of the saturation trace, and the reference streams
+    tr_sat = trace(n_sat, 51)
+    outs = {}
+    for k, m in set(tr_sat):
+        outs[(k, m)] = torch.empty(lscomp.max_compressed_bytes(pool[k][1]), dtype=torch.uint8, device="cuda")
+    size = torch.zeros(1, dtype=torch.int64, device="cuda")
+    lscomp.reserve(pool[4][1])
+
+    def plain():
+        for k, m in tr_sat:
+            lscomp.compress_async(pool[k][0], pool[k][1], *modes[m], outs[(k, m)], size)
+
+    plain()
+    torch.cuda.synchronize()
+    _, plain_ms, _ = timed_loop(torch, plain, 3)
+    ref = {}
+    for (k, m), o in outs.items():
+        _, n = lscomp.compress(pool[k][0], pool[k][1], *modes[m], out=o)
+        ref[(k, m)] = o[:n].cpu().numpy()
+    torch.cuda.synchronize()
+    res = {"frame_mix": "CSSI-style 512^2..2048^2, half lossless / half bins 3 5 10 15 p0.5",
+           "budget_bytes": budget,
+           "plain": {"requests": n_sat, "frames_per_s": n_sat / (plain_ms * 1e-3),
+                     "GBps": frame_bytes(tr_sat) / (plain_ms * 1e-3) / 1e9}}
+
+    def run(srv, tr, arrivals, check):
+        reqs = requests(tr)
+        t0 = time.perf_counter()
+        out = srv.replay(reqs, arrivals, hold=check)
+        wall = time.perf_counter() - t0
+        same = True
+        if check:
+            for c in out:
+                k, m = tr[c.tag]
+                r = ref.get((k, m))
+                if r is not None and c.status == lscomp.OK:
+                    got = _copy_dev(c.d_stream, c.stream_bytes)
+                    same = same and c.stream_bytes == r.size and np.array_equal(got, r)
+                srv.release(c.id)
+        ok = srv.drain() == lscomp.OK and all(c.status == lscomp.OK for c in out)
+        lat = np.array([c.t_done - c.t_submit for c in out]) * 1e3
+        st = srv.stats()
+        return {"requests": len(tr), "frames_per_s": len(tr) / wall, "GBps": frame_bytes(tr) / wall / 1e9,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "max_ms": float(lat.max()), "kernel_launches": st["launches"], "verified": bool(ok),
+                **({"identical_to_plain": bool(same)} if check else {})}
+
+    # device loop: offered rate (Poisson), then saturation (with a byte check of every stream)
+    n_off = int(seconds * rate)
+    tr_off = trace(n_off, 52)
+    arr = np.cumsum(np.random.default_rng(53).exponential(1.0 / rate, n_off))
+    srv = lscomp.Server.device(1 << 17, budget, idle_us=2000)
+    res["device_offered"] = {"offered_frames_per_s": rate, **run(srv, tr_off, arr, False)}
     srv.close()
-    lat = np.array([c.t_done - c.t_submit for c in out]) * 1e3
-    byts = sum(2 * pool[k][1][1] * pool[k][1][2] for k in pick)
-    return {"offered_frames_per_s": rate, "requests": n, "frames_per_s": n / wall,
-            "GBps": byts / wall / 1e9, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
-            "budget_bytes": 64 << 30, "slots": 4, "verified": bool(ok)}
+    srv = lscomp.Server.device(1 << 17, budget, idle_us=2000)
+    run(srv, tr_sat[:200], np.zeros(200), False)   # warm-up
+    res["device_saturation"] = run(srv, tr_sat, np.zeros(n_sat), True)
+    srv.close()
+    srv = lscomp.Server(slots=4, max_items=256, max_blocks=1 << 22, budget_bytes=budget)
+    run(srv, tr_sat[:200], np.zeros(200), False)
+    res["host_saturation"] = run(srv, tr_sat, np.zeros(n_sat), True)
+    srv.close()
+    res["device_vs_plain_frames_per_s"] = res["device_saturation"]["frames_per_s"] / res["plain"]["frames_per_s"]
+    return res
+
+
+def _copy_dev(ptr, nbytes):
+    """Copy a stream out of the server's arena (a raw device pointer) to the host."""
+    import ctypes
+    import numpy as np
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    host = np.empty(nbytes, np.uint8)
+    if hip.hipMemcpy(host.ctypes.data, ptr, nbytes, 2) != 0:   # hipMemcpyDeviceToHost
+        raise RuntimeError("hipMemcpy failed")
+    return host
 
 
 def bench_u32(torch, lscomp, d_in16, dims, d_out16, args):
@@ -212,10 +398,11 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=48, help="frames of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and served measurements")
-    ap.add_argument("--modes", default="lossless,lossy,u32",
-                    help="which settings to time (lossless,lossy; u32: the same volume as uint32)")
+    ap.add_argument("--modes", default="lossless,lossy,xpcs,small,u32",
+                    help="settings to time: lossless,lossy (CSSI volume), xpcs, small, u32")
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--served-rate", type=float, default=20000.0)
+    ap.add_argument("--served-saturation", type=int, default=8000, help="requests of the saturation runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; use --gpus 1", file=sys.stderr)
@@ -230,6 +417,7 @@ def main():
     n_elem = dims[0] * dims[1] * dims[2]
     in_bytes = 2 * n_elem
     torch.cuda.set_device(0)
+    wanted = args.modes.split(",")
 
     t = time.perf_counter()
     vol = datagen.volume("cssi", 2, dims)
@@ -243,40 +431,11 @@ def main():
     lscomp.reserve(dims)
 
     results = {}
-    settings = [("lossless", (1, 1, 1, 1), 1.0), ("lossy_b3-5-10-15_p0.5", (3, 5, 10, 15), 0.5)]
-    wanted = args.modes.split(",")
-    settings = [st for st in settings if st[0].split("_")[0] in wanted]
-    for name, bins, pool in settings:
-        comp = lambda: lscomp.compress_async(d_in, dims, bins, pool, d_out, d_size)
-        for _ in range(args.warmup):
-            comp()
-        torch.cuda.synchronize()
-        c_wall, c_ev, _ = timed_loop(torch, comp, args.steps)
-        s_bytes = int(d_size.item())
-        dec = lambda: lscomp.decompress_async(d_out, s_bytes, dims, bins, d_dec)
-        for _ in range(args.warmup):
-            dec()
-        torch.cuda.synchronize()
-        d_wall, d_ev, _ = timed_loop(torch, dec, args.steps)
-        st = lscomp.stream_status()
-        ok = st == lscomp.OK
-        if name == "lossless":
-            ok = ok and bool(torch.equal(d_dec, d_in))
-        else:
-            diff = ((d_in.to(torch.int32) & 0xFFFF) - (d_dec.to(torch.int32) & 0xFFFF)).abs()
-            results_max_err = int(diff.max().item())
-        results[name] = {
-            "compress_GBps": in_bytes / (c_wall * 1e-3) / 1e9,
-            "decompress_GBps": in_bytes / (d_wall * 1e-3) / 1e9,
-            "compress_ms": c_wall, "decompress_ms": d_wall,
-            "compress_kernel_ms": c_ev, "decompress_kernel_ms": d_ev,
-            "stream_bytes": s_bytes, "ratio": in_bytes / s_bytes,
-            "compress_hbm_frac": (in_bytes + s_bytes) / (c_ev * 1e-3) / HBM_PEAK,
-            "decompress_hbm_frac": (in_bytes + s_bytes) / (d_ev * 1e-3) / HBM_PEAK,
-            "verified": ok,
-        }
-        if name != "lossless":
-            results[name]["max_abs_err"] = results_max_err
+    for name, bins, pool in (("lossless", (1, 1, 1, 1), 1.0), ("lossy_b3-5-10-15_p0.5", (3, 5, 10, 15), 0.5)):
+        if name.split("_")[0] not in wanted:
+            continue
+        results[name] = time_codec(torch, lscomp, d_in, dims, bins, pool, d_out, d_dec, d_size, args, in_bytes,
+                                   bins == (1, 1, 1, 1))
         log(f"{name}: {json.dumps(results[name])}")
 
     if "u32" in wanted:
@@ -296,12 +455,18 @@ def main():
         for d in ("compress", "decompress"):
             r[f"{d}_copy_frac"] = r[f"{d}_hbm_frac"] * (HBM_PEAK / 1e9) / copy_gbps
 
+    del d_dec
+    if "xpcs" in wanted:
+        results.update(bench_xpcs(torch, lscomp, datagen, np, args))
+    if "small" in wanted:
+        results.update(bench_small(torch, lscomp, datagen, np, args))
+
     forms = {}
     if not args.no_forms:
-        del d_dec
         forms["batched"] = bench_batched(torch, lscomp, datagen, np, max(3, args.steps // 2))
         log(f"batched: {json.dumps(forms['batched'])}")
-        forms["served"] = bench_served(torch, lscomp, datagen, np, args.served_seconds, args.served_rate)
+        forms["served"] = bench_served(torch, lscomp, datagen, np, args.served_seconds, args.served_rate,
+                                       args.served_saturation)
         log(f"served: {json.dumps(forms['served'])}")
 
     ll = results["lossless"]
@@ -309,13 +474,8 @@ def main():
     achieved = (in_bytes + s_bytes) / (ll["compress_kernel_ms"] * 1e-3)
     cpu = None
     if not args.no_cpu:
-        frames = min(args.cpu_frames, dims[0])
-        gbps, threads, nbytes, dt, gbps1, frames1 = cpu_baseline(vol, frames, (1, 1, 1, 1), 1.0)
-        cpu = {"value": gbps, "unit": "GB/s", "cores": threads, "kind": "port",
-               "sample": f"lossless compression of the first {frames} frames ({nbytes / 1e9:.3f} GB), "
-                         f"OpenMP two-pass oracle, {dt:.3f} s per pass",
-               "value_1_thread": gbps1,
-               "sample_1_thread": f"serial oracle, first {frames1} frames"}
+        cpu = cpu_baseline(vol, (1, 1, 1, 1), 1.0, frames_all=min(args.cpu_frames, dims[0]))
+        log(f"cpu: {json.dumps(cpu)}")
     rec = {
         "metric": "compress/decompress GB/s + % of HBM peak on 1 MI355X; p99 per-frame latency",
         "value": ll["compress_GBps"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,

@@ -130,22 +130,24 @@ int lscomp_get_launch_shape_ex(int direction, lscomp_launch_shape *shape);
  * for an invalid mode).  Output is identical in every mode. */
 int lscomp_set_decode_prepass(int mode);
 
-/* One measured launch shape (times in ms, averaged over the repetitions). */
+/* One measured launch shape (ms per call, averaged over the repetitions). */
 typedef struct lscomp_tune_entry {
-    uint32_t tile, wg_per_cu;
-    float compress_ms, decompress_ms;
+    uint32_t direction;        /* 0 compression, 1 decompression */
+    uint32_t tile, wg_per_cu, lb_window, counters;
+    float ms;
 } lscomp_tune_entry;
 
-/* Autotuner: times compression and decompression of the caller's volume
- * (d_in, dims, bins, pooling) for every launch shape (tile 128/256/512 x
- * workgroups per CU 1..4 and the occupancy limit), using d_stream (capacity
- * >= lscomp_max_compressed_bytes_u16(dims)) and d_decoded (same size as the
- * volume) as work buffers, then applies the fastest shape of each direction
- * with lscomp_set_launch_shape.  Results are cached per (size class, mode):
- * a later call for the same class applies the cached shapes without timing.
- * If table/n_table are given, the measured entries are copied there (up to
- * *n_table; *n_table is set to the number written, 0 for a cache hit).
- * No counterpart in the reference. */
+/* Autotuner: times compression and then decompression of the caller's volume
+ * (d_in, dims, bins, pooling) under launch shapes found by coordinate descent
+ * over tile (128/256/512), workgroups per CU (each value giving a distinct
+ * grid), look-back window and tile-id counters, from the current shape of
+ * each direction, using d_stream (capacity >= lscomp_max_compressed_bytes_u16
+ * (dims)) and d_decoded (same size as the volume) as work buffers; applies the
+ * fastest shape of each direction with lscomp_set_launch_shape_ex.  Results
+ * are cached per (size class, mode): a later call for the same class applies
+ * the cached shapes without timing.  If table/n_table are given, the measured
+ * entries are copied there (up to *n_table; *n_table is set to the number
+ * written, 0 for a cache hit).  No counterpart in the reference. */
 int lscomp_autotune_u16(const uint16_t *d_in, uint8_t *d_stream, size_t stream_capacity, uint16_t *d_decoded,
                         lscomp_dim3 dims, lscomp_bins4 bins, float pooling_th, int reps, hipStream_t stream,
                         lscomp_tune_entry *table, uint32_t *n_table);
@@ -172,12 +174,24 @@ typedef struct lscomp_item_u16 {
 typedef struct lscomp_plan_s *lscomp_plan;
 
 /* direction 0 = compress, 1 = decompress.  Capacity: up to max_items items and
- * max_blocks 8x8 blocks in total per batch.  Uses the launch shape of the
- * direction at creation time (lscomp_set_launch_shape). */
+ * max_blocks 8x8 blocks in total per batch.  Starts with the launch shape of
+ * the direction at creation time (lscomp_set_launch_shape[_ex]). */
 int lscomp_plan_create(int direction, uint32_t max_items, uint64_t max_blocks, lscomp_plan *plan);
 /* Validates the items and uploads the batch tables (stream-ordered on `stream`;
  * the host array may be reused when the call returns). */
 int lscomp_plan_set_items_u16(lscomp_plan plan, const lscomp_item_u16 *items, uint32_t n, hipStream_t stream);
+/* The plan's launch shape (tile, workgroups per CU, look-back window,
+ * counters; zeros take the direction's defaults).  Rebuilds the tables of the
+ * current batch when the tile changes and drops a captured graph (the next
+ * lscomp_plan_launch_graph captures again).  Streams are identical for every
+ * shape.  A plan starts with the direction's shape at creation. */
+int lscomp_plan_set_shape(lscomp_plan plan, const lscomp_launch_shape *shape, hipStream_t stream);
+int lscomp_plan_get_shape(lscomp_plan plan, lscomp_launch_shape *shape);
+/* Autotunes the plan's shape on its current batch (the same search as
+ * lscomp_autotune_u16; each measured launch rewrites the batch's outputs with
+ * the same bytes) and applies the fastest.  table / n_table as there. */
+int lscomp_plan_autotune(lscomp_plan plan, int reps, hipStream_t stream, lscomp_tune_entry *table,
+                         uint32_t *n_table);
 /* One persistent launch over every item of the current batch (stream-ordered). */
 int lscomp_plan_launch(lscomp_plan plan, hipStream_t stream);
 /* The same launch replayed from a hipGraph captured on first use (batches set

@@ -18,9 +18,11 @@ using namespace lsc;
 struct lscomp_plan_s {
     int device = 0;
     int direction = 0;            // 0 compress, 1 decompress
-    Shape shape{};                // launch shape, fixed at creation (a captured graph keeps it)
+    Shape shape{};                // launch shape (lscomp_plan_set_shape; a captured graph keeps it)
     uint32_t max_items = 0;
-    uint64_t max_tiles = 0;
+    uint64_t max_blocks = 0;
+    uint64_t max_tiles = 0;       // tile-table capacity, for the smallest tile (128 blocks)
+    std::vector<lscomp_item_u16> src;   // the current batch as the caller gave it
     uint32_t nitems = 0, ntiles = 0;
     // device
     Item *d_items = nullptr;
@@ -88,6 +90,84 @@ LaunchCfg plan_cfg(lscomp_plan p)
     return c;
 }
 
+// Caller holds p->mu.  Builds the item descriptors and the tile table of
+// p->src for the plan's tile size in the next staging buffer and uploads them
+// (stream-ordered).
+int upload_tables(lscomp_plan p, hipStream_t stream)
+{
+    const uint32_t n = (uint32_t)p->src.size();
+    // build the tables in the next staging buffer (wait until its last upload ran)
+    const int sb = p->stage_next;
+    int st = hip_fail(hipEventSynchronize(p->stage_done[sb]), "hipEventSynchronize");
+    if (st) return st;
+    Item *h_items = (Item *)p->h_stage[sb];
+    uint32_t *h_tiles = (uint32_t *)(p->h_stage[sb] + sizeof(Item) * p->max_items);
+    uint64_t tiles = 0, blocks = 0;
+    const bool comp = p->direction == 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const lscomp_item_u16 &src = p->src[i];
+        Item &it = h_items[i];
+        st = make_item(src.dims, src.bins, comp ? src.pooling_th : 0.0f, it, p->shape.tile);
+        if (st) return st;
+        if (!src.d_volume || !src.d_stream || ((uintptr_t)src.d_volume & 1) || ((uintptr_t)src.d_stream & 3))
+            return LSCOMP_ERR_INVALID_ARG;
+        if (src.stream_bytes < it.hdr_bytes) return comp ? LSCOMP_ERR_CAPACITY : LSCOMP_ERR_CORRUPT;
+        blocks += it.nblocks;
+        if (tiles + it.ntiles > p->max_tiles || blocks > p->max_blocks) return LSCOMP_ERR_TOO_LARGE;
+        it.first_tile = (uint32_t)tiles;
+        it.align = align_class(src.d_volume, src.dims.z);
+        it.s_len = src.stream_bytes;
+        it.result = p->d_sizes + i;
+        if (comp) {
+            it.vol_in = src.d_volume;
+            it.s_out = src.d_stream;
+        } else {
+            it.vol_out = src.d_volume;
+            it.s_in = src.d_stream;
+        }
+        for (uint32_t t = 0; t < it.ntiles; t++) h_tiles[tiles + t] = i;
+        tiles += it.ntiles;
+    }
+    h_tiles[p->max_tiles] = (uint32_t)tiles;   // staged tile count (copied to d_total)
+    st = hip_fail(hipMemcpyAsync(p->d_items, h_items, sizeof(Item) * n, hipMemcpyHostToDevice, stream), "upload");
+    if (st) return st;
+    st = hip_fail(hipMemcpyAsync(p->d_tile_items, h_tiles, sizeof(uint32_t) * tiles, hipMemcpyHostToDevice, stream),
+                  "upload");
+    if (st) return st;
+    st = hip_fail(hipMemcpyAsync(p->d_total, h_tiles + p->max_tiles, sizeof(uint32_t), hipMemcpyHostToDevice, stream),
+                  "upload");
+    if (st) return st;
+    st = hip_fail(hipEventRecord(p->stage_done[sb], stream), "hipEventRecord");
+    if (st) return st;
+    p->stage_next = sb ^ 1;
+    p->nitems = n;
+    p->ntiles = (uint32_t)tiles;
+    return LSCOMP_OK;
+}
+
+// Caller holds p->mu.  A new shape: a captured graph is dropped (it holds the
+// old grid and kernel), and the tables are rebuilt when the tile size changed.
+int apply_shape(lscomp_plan p, const Shape &sh, hipStream_t stream)
+{
+    const bool retile = sh.tile != p->shape.tile;
+    const Shape old = p->shape;
+    p->shape = sh;
+    if (p->exec) {
+        (void)hipGraphExecDestroy(p->exec);
+        (void)hipGraphDestroy(p->graph);
+        p->exec = nullptr;
+        p->graph = nullptr;
+    }
+    if (retile && !p->src.empty()) {
+        const int st = upload_tables(p, stream);
+        if (st) {
+            p->shape = old;
+            return st;
+        }
+    }
+    return LSCOMP_OK;
+}
+
 hipError_t plan_kernel(lscomp_plan p, hipStream_t stream)
 {
     Item unused;
@@ -109,8 +189,10 @@ int lscomp_plan_create(int direction, uint32_t max_items, uint64_t max_blocks, l
     p->direction = direction;
     p->shape = load_shape(direction);
     p->max_items = max_items;
-    // every item wastes at most one partial tile
-    p->max_tiles = (max_blocks + p->shape.tile - 1) / p->shape.tile + max_items;
+    p->max_blocks = max_blocks;
+    // tables for the smallest tile, so the shape can change later; every item
+    // wastes at most one partial tile
+    p->max_tiles = (max_blocks + 127) / 128 + max_items;
     if (p->max_tiles > 0xffffffffull) { delete p; return LSCOMP_ERR_TOO_LARGE; }
     if (hip_fail(hipGetDevice(&p->device), "hipGetDevice")) { delete p; return LSCOMP_ERR_HIP; }
     p->stage_bytes = table_bytes(max_items, p->max_tiles);
@@ -140,51 +222,32 @@ int lscomp_plan_set_items_u16(lscomp_plan p, const lscomp_item_u16 *items, uint3
 {
     if (!p || !items || n == 0 || n > p->max_items) return LSCOMP_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(p->mu);
-    // build the tables in the next staging buffer (wait until its last upload ran)
-    const int sb = p->stage_next;
-    int st = hip_fail(hipEventSynchronize(p->stage_done[sb]), "hipEventSynchronize");
+    std::vector<lscomp_item_u16> prev;
+    prev.swap(p->src);
+    p->src.assign(items, items + n);
+    const int st = upload_tables(p, stream);
+    if (st) p->src.swap(prev);
+    return st;
+}
+
+int lscomp_plan_set_shape(lscomp_plan p, const lscomp_launch_shape *shape, hipStream_t stream)
+{
+    if (!p || !shape) return LSCOMP_ERR_INVALID_ARG;
+    Shape sh{shape->tile, shape->wg_per_cu, shape->lb_window, shape->counters};
+    int st = normalize_shape(p->direction, sh);
     if (st) return st;
-    Item *h_items = (Item *)p->h_stage[sb];
-    uint32_t *h_tiles = (uint32_t *)(p->h_stage[sb] + sizeof(Item) * p->max_items);
-    uint64_t tiles = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const lscomp_item_u16 &src = items[i];
-        Item &it = h_items[i];
-        const bool comp = p->direction == 0;
-        st = make_item(src.dims, src.bins, comp ? src.pooling_th : 0.0f, it, p->shape.tile);
-        if (st) return st;
-        if (!src.d_volume || !src.d_stream || ((uintptr_t)src.d_volume & 1) || ((uintptr_t)src.d_stream & 3))
-            return LSCOMP_ERR_INVALID_ARG;
-        if (src.stream_bytes < it.hdr_bytes) return comp ? LSCOMP_ERR_CAPACITY : LSCOMP_ERR_CORRUPT;
-        if (tiles + it.ntiles > p->max_tiles) return LSCOMP_ERR_TOO_LARGE;
-        it.first_tile = (uint32_t)tiles;
-        it.align = align_class(src.d_volume, src.dims.z);
-        it.s_len = src.stream_bytes;
-        it.result = p->d_sizes + i;
-        if (comp) {
-            it.vol_in = src.d_volume;
-            it.s_out = src.d_stream;
-        } else {
-            it.vol_out = src.d_volume;
-            it.s_in = src.d_stream;
-        }
-        for (uint32_t t = 0; t < it.ntiles; t++) h_tiles[tiles + t] = i;
-        tiles += it.ntiles;
-    }
-    h_tiles[p->max_tiles] = (uint32_t)tiles;   // staged tile count (copied to d_total)
-    st = hip_fail(hipMemcpyAsync(p->d_items, h_items, sizeof(Item) * n, hipMemcpyHostToDevice, stream), "upload");
-    if (st) return st;
-    st = hip_fail(hipMemcpyAsync(p->d_tile_items, h_tiles, sizeof(uint32_t) * tiles, hipMemcpyHostToDevice, stream),
-                  "upload");
-    if (st) return st;
-    st = hip_fail(hipMemcpyAsync(p->d_total, h_tiles + p->max_tiles, sizeof(uint32_t), hipMemcpyHostToDevice, stream),
-                  "upload");
-    if (st) return st;
-    st = hip_fail(hipEventRecord(p->stage_done[sb], stream), "hipEventRecord");
-    if (st) return st;
-    p->stage_next = sb ^ 1;
-    p->nitems = n;
-    p->ntiles = (uint32_t)tiles;
+    std::lock_guard<std::mutex> lk(p->mu);
+    return apply_shape(p, sh, stream);
+}
+
+int lscomp_plan_get_shape(lscomp_plan p, lscomp_launch_shape *shape)
+{
+    if (!p || !shape) return LSCOMP_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(p->mu);
+    shape->tile = p->shape.tile;
+    shape->wg_per_cu = p->shape.wgpcu;
+    shape->lb_window = p->shape.win;
+    shape->counters = p->shape.nc;
     return LSCOMP_OK;
 }
 
@@ -225,6 +288,37 @@ int lscomp_plan_launch_graph(lscomp_plan p, hipStream_t stream)
     int st = count_launch(p, stream);
     if (st) return st;
     return hip_fail(hipGraphLaunch(p->exec, stream), "hipGraphLaunch");
+}
+
+int lscomp_plan_autotune(lscomp_plan p, int reps, hipStream_t stream, lscomp_tune_entry *table, uint32_t *n_table)
+{
+    if (!p || reps < 1) return LSCOMP_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->nitems == 0) return LSCOMP_ERR_INVALID_ARG;
+    const Shape saved = p->shape;
+    std::vector<lscomp_tune_entry> log;
+    auto measure = [&](const Shape &s) -> float {
+        if (apply_shape(p, s, stream) != LSCOMP_OK) return -1.0f;
+        int rs = LSCOMP_OK;
+        const float ms = time_launches(stream, reps, [&] {
+            const int c = count_launch(p, stream);
+            return c ? c : hip_fail(plan_kernel(p, stream), "plan launch");
+        }, &rs);
+        return rs == LSCOMP_OK ? ms : -1.0f;
+    };
+    Shape best{};
+    int st = tune_shape(p->direction, true, saved, measure, best, log);
+    if (!st) st = apply_shape(p, best, stream);
+    if (st) {
+        (void)apply_shape(p, saved, stream);
+        return st;
+    }
+    if (table && n_table) {
+        uint32_t k = 0;
+        for (; k < *n_table && k < log.size(); k++) table[k] = log[k];
+        *n_table = k;
+    }
+    return LSCOMP_OK;
 }
 
 const uint64_t *lscomp_plan_device_sizes(lscomp_plan p) { return p ? p->d_sizes : nullptr; }

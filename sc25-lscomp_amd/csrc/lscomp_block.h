@@ -187,20 +187,37 @@ enum : uint32_t {
     DEC_Q24 = 8u,       // every bin in [2, 16384]: 24-bit multiply-high division
 };
 
+// Compile-time specialization of the encoder for a known kind of settings
+// (the caller checked p.flags): only that kind's code is compiled, which
+// keeps the general case's scalar registers out of the hot loop.
+enum : int {
+    KIND_ANY = 0,
+    KIND_LOSSLESS = 1,   // DEC_LOSSLESS
+    KIND_Q24 = 2,        // DEC_Q24 (lossy, no bin of 1)
+};
+template <int kKind>
+LSC_HD uint32_t kind_flags(const Decide &p)
+{
+    if (kKind == KIND_LOSSLESS) return DEC_LOSSLESS;
+    if (kKind == KIND_Q24) return (p.flags & (DEC_POOL | DEC_BINSEL)) | DEC_Q24;
+    return p.flags;
+}
+
 // Header byte of a block: (pool << 7) | (bin_choice << 5) | rate.  kMem: p
-// lives in device memory (see sel4).
-template <bool kMem = false>
+// lives in device memory (see sel4).  kKind: KIND_*.
+template <bool kMem = false, int kKind = KIND_ANY>
 LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
 {
+    const uint32_t flags = kind_flags<kKind>(p);
     // max of all 64 values (packed 16-bit max)
     u16x2 mx = as_u16x2(D[0]);
 #pragma unroll
     for (int i = 1; i < 32; i++) mx = __builtin_elementwise_max(mx, as_u16x2(D[i]));
     uint32_t m1 = mx.x > mx.y ? mx.x : mx.y;
-    if (p.flags & DEC_LOSSLESS) return (3u << 5) | bit_width(m1);
+    if (flags & DEC_LOSSLESS) return (3u << 5) | bit_width(m1);
 
     uint32_t pool = 0, s2 = 0;
-    if (p.flags & DEC_POOL) {
+    if (flags & DEC_POOL) {
         // nonzero count: min(v,1) per half, two packed pairs summed per 32-bit
         // three-input add (no carry between the halves: each stays <= 32);
         // max pair sum
@@ -225,7 +242,7 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
     }
 
     uint32_t bc = 3;
-    if (p.flags & DEC_BINSEL) {
+    if (flags & DEC_BINSEL) {
         // bin choice: bc = #{k in 1..3 : m0 >= b[k]}, m0 = min{v : v >= b0}
         // (no value in [b0,b1) <=> m0 >= b1; bins are sorted).  When m1 < b1
         // the maximum decides it: no value >= b0 (bc = 3), or m0 <= m1 < b1
@@ -245,7 +262,7 @@ LSC_HD uint32_t block_header(const uint32_t D[32], const Decide &p)
     }
     // max floor(floor((a+b)/2) / bin) = floor(max(a+b) / 2bin)
     uint32_t maxq;
-    if (!kMem && (p.flags & DEC_Q24)) {
+    if (!kMem && (flags & DEC_Q24)) {
         const uint32_t sc = pool ? sel4<kMem>(p.ps, bc) & 0xffffu : sel4<kMem>(p.qs, bc);
         maxq = mulhi24((pool ? s2 : m1) * sc, pool ? sel4<kMem>(p.pm, bc) : sel4<kMem>(p.qm, bc));
     } else {
@@ -388,16 +405,19 @@ LSC_HD void pooled_columns(const uint32_t Q[32], uint32_t C[8], bool hi)
 // Encode the payload of one block into words out[0..]; returns the number of
 // 32-bit words (2*rate plain, rate pooled).  hdr from block_header().
 // Quantizes D in place for plain blocks.  kMem: p lives in device memory.
-template <bool kMem = false>
+// kKind: as for block_header.
+template <bool kMem = false, int kKind = KIND_ANY>
 LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint32_t *out)
 {
     const uint32_t rate = hdr & 0x1fu;
     if (rate == 0) return 0;
+    const uint32_t flags = kind_flags<kKind>(p);
     const uint32_t bc = (hdr >> 5) & 3u;
-    const bool identity = sel4<kMem>(p.b, bc) == 1u;
-    if (hdr & 0x80u) {
+    const bool lossless = kKind == KIND_LOSSLESS;
+    const bool identity = lossless || sel4<kMem>(p.b, bc) == 1u;
+    if (!lossless && (hdr & 0x80u)) {
         uint32_t Q[32], C[8];
-        if (!kMem && (p.flags & DEC_Q24)) quantize_pooled24(D, sel4<kMem>(p.ps, bc), sel4<kMem>(p.pm, bc), Q);
+        if (!kMem && (flags & DEC_Q24)) quantize_pooled24(D, sel4<kMem>(p.ps, bc), sel4<kMem>(p.pm, bc), Q);
         else quantize_pooled(D, sel4<kMem>(p.f2, bc), Q);
         pooled_columns(Q, C, false);
         const uint32_t nlo = rate < 8 ? rate : 8u;
@@ -408,8 +428,12 @@ LSC_HD uint32_t encode_block(uint32_t D[32], uint32_t hdr, const Decide &p, uint
         }
         return rate;
     }
-    if (!kMem && (p.flags & DEC_Q24)) quantize_plain24(D, sel4<kMem>(p.qs, bc), sel4<kMem>(p.qm, bc));   // no bin is 1
-    else if (!identity) quantize_plain(D, sel4<kMem>(p.f, bc));
+    if (lossless) {
+    } else if (!kMem && (flags & DEC_Q24)) {
+        quantize_plain24(D, sel4<kMem>(p.qs, bc), sel4<kMem>(p.qm, bc));   // no bin is 1
+    } else if (!identity) {
+        quantize_plain(D, sel4<kMem>(p.f, bc));
+    }
     uint32_t C[16];
     columns_from_rows(D, C, false);
     const uint32_t nlo = rate < 8 ? rate : 8u;

@@ -2,6 +2,8 @@
 // batched (lscomp_batch.cpp) entry points.
 #pragma once
 #include <atomic>
+#include <functional>
+#include <vector>
 #include <stdint.h>
 
 #include "../../include/lscomp.h"
@@ -27,6 +29,13 @@ int make_item(lscomp_dim3 dims, lscomp_bins4 bins, float p, Item &it, uint32_t t
 // Stream-ordered copy of a plan's error word (ERRF_*) into pinned host memory,
 // then cleared on the device (lscomp_batch.cpp).
 int plan_take_error_async(lscomp_plan p, uint32_t *h_err, hipStream_t stream);
+// Autotuner (lscomp_tune.cpp): the average ms of `reps` calls of fn on stream
+// after one warm-up (negative if a call fails; its status in *status), and a
+// coordinate descent over the shape axes from `start` with `measure` (ms of a
+// shape, negative on failure), logging every measured shape.
+float time_launches(hipStream_t stream, int reps, const std::function<int()> &fn, int *status);
+int tune_shape(int direction, bool table, Shape start, const std::function<float(const Shape &)> &measure,
+               Shape &best, std::vector<lscomp_tune_entry> &log);
 // Grid and scheduling of a launch of `total_tiles` tiles with a normalized shape.
 LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, const Shape &shape);
 

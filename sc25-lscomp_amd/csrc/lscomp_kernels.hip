@@ -295,8 +295,9 @@ __device__ __forceinline__ void store_block_u32(const Item &it, uint32_t bx, uin
 // What the tile kernels need of a value type: registers per block, payload
 // words per block at most, and the per-block load, header, encode, decode and
 // store.  kMem: the item descriptors live in device memory (table mode), so
-// per-block parameters are read with indexed loads (sel4).
-template <bool kMem>
+// per-block parameters are read with indexed loads (sel4).  kKind: KIND_* of
+// every item (block_header).
+template <bool kMem, int kKind = KIND_ANY>
 struct CodecU16T {
     static constexpr int kRegs = 32;              // 64 values as packed pairs
     static constexpr uint32_t kMaxWords = 32;     // 16 planes x 8 bytes
@@ -306,11 +307,11 @@ struct CodecU16T {
     }
     __device__ __forceinline__ static uint32_t header(const uint32_t R[kRegs], const Item &it, Ctrl *)
     {
-        return block_header<kMem>(R, it.dec);
+        return block_header<kMem, kKind>(R, it.dec);
     }
     __device__ __forceinline__ static void encode(uint32_t R[kRegs], uint32_t hdr, const Item &it, uint32_t *out)
     {
-        encode_block<kMem>(R, hdr, it.dec, out);
+        encode_block<kMem, kKind>(R, hdr, it.dec, out);
     }
     __device__ __forceinline__ static bool bad_header(uint32_t hdr) { return (hdr & 0x1fu) > 16u; }
     __device__ __forceinline__ static void decode(const uint32_t *in, uint32_t hdr, const Item &it, uint32_t R[kRegs])
@@ -664,13 +665,20 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
 // pending tiles are lower), so one of them takes it.  Only where no tile
 // waits for another (decode with precomputed offsets) are ids taken ahead: a
 // reserved tile must never hold up a look-back.
+//
+// take() returns the counter's raw value k and id(k) the tile: a value
+// computed from the atomic's result at once would make the wave wait for its
+// round trip, and gfx9 counts loads, stores and atomics in one in-order
+// counter, so that wait would also cover every access issued before it.  The
+// raw value goes to LDS where the old tile was written anyway (the wait falls
+// where the loads it covers have returned) and is converted after the barrier.
 struct TileQ {
     uint32_t cls, nc;
     __device__ __forceinline__ uint32_t take(Ctrl *ctrl) const
     {
-        const uint32_t k = __hip_atomic_fetch_add(&ctrl->ctr[cls].v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return k * nc + cls;
+        return __hip_atomic_fetch_add(&ctrl->ctr[cls].v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __device__ __forceinline__ uint32_t id(uint32_t k) const { return k * nc + cls; }
 };
 
 // Thread 0: the workgroup's class (one ticket atomic when there are several
@@ -804,10 +812,11 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
     const TileQ q = uniform_queue(sh.cls, sc);
+    const uint32_t n0 = q.id(sh.next);
 
     uint32_t np = 0, hb = 0;   // pending tiles (p0 oldest), header buffer of the next tile
     PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
-    uint32_t n = sh.next, par = 0;   // par: iteration parity (LDS buffers written before the scan barrier)
+    uint32_t n = n0, par = 0;   // par: iteration parity (LDS buffers written before the scan barrier)
     for (;; par ^= 1u) {
         const bool have_n = n < ntot;
         if (!have_n && np == 0) break;
@@ -835,9 +844,10 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
             lb_probe<TILE>(status, (int64_t)p0.tile - 1, item_at<kTable>(single, items, p0.item).first_tile, epoch,
                            win, w0);
         // Issued last: returns are counted in order, so waiting for the loads or
-        // the probe must not include the atomic's longer round trip.
-        uint32_t next_id = n;
-        if (tid == 0 && have_n) next_id = q.take(ctrl);
+        // the probe must not include the atomic's longer round trip (the raw
+        // value is only written to LDS before the scan barrier).
+        uint32_t next_k = ~0u;
+        if (tid == 0 && have_n) next_k = q.take(ctrl);
 
         // 2. header of n; n's scan and the first half of p0's evaluation share a barrier
         uint32_t n_hdr = 0, n_words = 0;
@@ -851,9 +861,10 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
         const uint32_t incl = wave_incl_scan(n_words, lane);
         if (lane == 63) sh.wsum[par][wave] = incl;
         if (p_lb) lb_slices<TILE>(w0, epoch, win, sh.lbf, par);
-        if (tid == 0) sh.next2[par] = next_id;
+        if (tid == 0) sh.next2[par] = next_k;
         __syncthreads();
-        const uint32_t n_next = sh.next2[par];
+        const uint32_t next_raw = sh.next2[par];
+        const uint32_t n_next = next_raw == ~0u ? n : q.id(next_raw);
         uint32_t before = 0, n_agg = 0;
 #pragma unroll
         for (int v = 0; v < TILE / 64; v++) {
@@ -917,13 +928,16 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
     finish_launch(ctrl, epoch);
 }
 
-template <int TILE, bool kTable>
+// kKind: KIND_* of every item.  The general kernel's code for every kind of
+// settings costs scalar register spills: lossless CSSI compression was 2-3 %
+// slower in it than in its own kernel.
+template <int TILE, bool kTable, int kKind = KIND_ANY>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void compress_kernel(
     const Item single, const Item *__restrict__ items, const uint32_t *__restrict__ tile_items, uint32_t total_tiles,
     const uint32_t *d_total_tiles, Ctrl *ctrl, uint64_t *status, Sched sc)
 {
-    compress_tiles<CodecU16T<kTable>, TILE, kTable>(single, items, tile_items, total_tiles, d_total_tiles, ctrl, status,
-                                                    sc);
+    compress_tiles<CodecU16T<kTable, kKind>, TILE, kTable>(single, items, tile_items, total_tiles, d_total_tiles,
+                                                               ctrl, status, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -970,21 +984,51 @@ __device__ __forceinline__ uint32_t words16(const Item &it, uint64_t off)
 // launch (its chunk is still being resolved); on a timeout the launch is
 // flagged and an offset past any stream is returned, so the tile decodes to
 // zeros.
-__device__ __forceinline__ uint64_t tile_offset(const uint64_t *tile_off, uint32_t u, uint32_t epoch, Ctrl *ctrl)
+// The word is loaded early with a plain load (off_word) and checked only
+// where the offset is used (off_value), so the check never waits for a load
+// in flight; a stale cached copy carries another epoch and is read again.
+// The early load is a scalar load (through the constant address space): its
+// wait counts scalar loads and LDS operations only.  A vector load of the
+// uniform word is moved into scalar registers where it is issued, or copied
+// between loop iterations, and on gfx9 either waits for every vector memory
+// operation in flight, the previous tile's stores included (uint32 decode
+// 1.49 -> 1.98 ms).  The scalar cache does not see other workgroups' writes
+// made during the launch: such a stale word fails the epoch check and is read
+// again with a vector load in off_value.
+typedef const __attribute__((address_space(4))) uint64_t const_u64;
+__device__ __forceinline__ uint64_t off_word(const uint64_t *tile_off, uint32_t u)
+{
+    return ((const_u64 *)tile_off)[u];
+}
+
+// The offset (uniform: every lane checked the same word) in scalar registers.
+__device__ __forceinline__ uint64_t off_value(const uint64_t *tile_off, uint32_t u, uint64_t w, uint32_t epoch,
+                                              Ctrl *ctrl)
 {
     const uint32_t want = (epoch << 2) | LB_INCL;   // bits 36.. of the word
-    uint64_t w = ld_relaxed(tile_off + u);
-    for (uint32_t spins = 1; (uint32_t)(w >> 36) != want; spins++) {
+    uint64_t v;
+    for (uint32_t spins = 1;; spins++) {
+        if ((uint32_t)(w >> 36) == want) {
+            v = w & kValMask;
+            break;
+        }
         if (spins > kSpinLimit ||
             ((spins & 63u) == 0 &&
              (__hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERRF_TIMEOUT))) {
             __hip_atomic_fetch_or(&ctrl->error, ERRF_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return ~0ull >> 3;
+            v = ~0ull >> 3;
+            break;
         }
         __builtin_amdgcn_s_sleep(1);
         w = ld_relaxed(tile_off + u);
     }
-    return w & kValMask;
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+__device__ __forceinline__ uint64_t tile_offset(const uint64_t *tile_off, uint32_t u, uint32_t epoch, Ctrl *ctrl)
+{
+    return off_value(tile_off, u, off_word(tile_off, u), epoch, ctrl);
 }
 
 // First phase of a single-volume decode launch: the payload offset of every
@@ -1061,8 +1105,8 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     if (tid == 0) {
         sh.cls = tile_class(ctrl, sc);
         const TileQ q0{sh.cls, sc.nc};
-        sh.tile = q0.take(ctrl);
-        if (tile_off) sh.next = q0.take(ctrl);
+        sh.tile = q0.id(q0.take(ctrl));
+        if (tile_off) sh.next = q0.take(ctrl);   // raw (TileQ)
     }
     __syncthreads();
     const TileQ q = uniform_queue(sh.cls, sc);
@@ -1074,19 +1118,19 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     // is safe because no tile waits for another.  With the look-back, ids are
     // taken one at a time.
     uint32_t hdr = 0;
-    uint64_t excl_next = 0;
+    uint64_t w_next = 0;   // offset word of the next tile, checked when used
     if (tile_off && sh.tile < ntot) {
         const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
         const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
         hdr = b < it.nblocks ? (uint32_t)it.s_in[b] : 0u;
-        excl_next = tile_offset(tile_off, sh.tile, epoch, ctrl);
+        w_next = off_word(tile_off, sh.tile);
     }
 
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
-        const uint32_t ntile = tile_off ? sh.next : ntot;
-        uint32_t after_next = 0;   // thread 0: the id two tiles ahead
+        const uint32_t ntile = tile_off ? q.id(sh.next) : ntot;
+        uint32_t after_next = 0;   // thread 0: the raw id two tiles ahead
         if (tile_off && tid == 0) after_next = q.take(ctrl);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
@@ -1094,7 +1138,8 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         const bool valid = b < it.nblocks;
         if (!tile_off) hdr = valid ? (uint32_t)it.s_in[b] : 0u;
 
-        uint64_t excl = excl_next;   // with precomputed tile offsets no look-back is needed
+        // with precomputed tile offsets no look-back is needed
+        uint64_t excl = tile_off ? off_value(tile_off, tile, w_next, epoch, ctrl) : 0ull;
         const uint32_t words = header_payload_bytes(hdr) >> 2;
         bool bad = (hdr & 0x1fu) > 16u;
 
@@ -1119,7 +1164,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
             const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
             hdr = nb < nit.nblocks ? (uint32_t)nit.s_in[nb] : 0u;
-            if (tile_off) excl_next = tile_offset(tile_off, ntile, epoch, ctrl);
+            if (tile_off) w_next = off_word(tile_off, ntile);
         }
         __syncthreads();
 
@@ -1144,7 +1189,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
                 sh.tile = ntile;
                 sh.next = after_next;
             } else {
-                sh.tile = q.take(ctrl);
+                sh.tile = q.id(q.take(ctrl));   // no tile is taken ahead here
             }
         }
         __syncthreads();
@@ -1181,7 +1226,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
     __syncthreads();
     const uint32_t epoch = sh.epoch;
     if (phase) offset_phase<TILE>(it, ctrl, status, tile_off, epoch, sc.win, sh);
-    uint32_t pend = 0;   // thread 0: the id of the tile two ahead of the next one
+    uint32_t pend = 0;   // thread 0: the raw id of the tile two ahead of the next one (TileQ)
     if (tid == 0) {
         sh.cls = tile_class(ctrl, sc);
         const TileQ q0{sh.cls, sc.nc};
@@ -1191,7 +1236,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
     }
     __syncthreads();
     const TileQ q = uniform_queue(sh.cls, sc);
-    uint32_t t = sh.tile, t1 = sh.next;
+    uint32_t t = q.id(sh.tile), t1 = q.id(sh.next);
 
     // header byte of this thread's block in tile u, and u's payload byte offset
     const auto hdr_of = [&](uint32_t u) -> uint32_t {
@@ -1201,10 +1246,15 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
     const auto g0_of = [&](uint32_t u) -> uint64_t {
         return u < ntot ? (uint64_t)it.hdr_bytes + 4ull * tile_offset(tile_off, u, epoch, ctrl) : 0ull;
     };
+    // from a word loaded earlier (off_word), checked now
+    const auto g0_from = [&](uint32_t u, uint64_t w) -> uint64_t {
+        return u < ntot ? (uint64_t)it.hdr_bytes + 4ull * off_value(tile_off, u, w, epoch, ctrl) : 0ull;
+    };
 
-    // tile t: scan, then its payload staged at the start of the ring
+    // tile t: scan, then its payload staged at the start of the ring; t1's
+    // offset word is checked where its payload loads need it
     uint32_t hdr = hdr_of(t), hdr1 = hdr_of(t1);
-    uint64_t g0 = g0_of(t), g1 = g0_of(t1);
+    uint64_t g0 = g0_of(t), w1 = t1 < ntot ? off_word(tile_off, t1) : 0ull;
     uint32_t agg = 0, t_start = 0;
     uint32_t off = tile_scan<TILE>(header_payload_bytes(hdr) >> 2, tid, sh.wsum[0], &agg);
     bool fits = agg <= kMax && g0 + 4ull * agg <= it.s_len;
@@ -1218,7 +1268,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
         if (lane == 63) sh.wsum[0][wave] = incl;
         if (tid == 0) sh.next = pend;
         __syncthreads();
-        const uint32_t t2 = __builtin_amdgcn_readfirstlane(sh.next);
+        const uint32_t t2 = q.id(__builtin_amdgcn_readfirstlane(sh.next));
         uint32_t before = 0, agg1 = 0;
 #pragma unroll
         for (int v = 0; v < TILE / 64; v++) {
@@ -1229,10 +1279,11 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
         const uint32_t off1 = before + incl - words1;
         if (tid == 0) pend = q.take(ctrl);
 
-        // 2. t2's header byte and offset, t1's payload loads (in this order: the
-        //    wait for the payload in step 4 then covers all of them)
+        // 2. t2's header byte and offset word, t1's payload loads (in this
+        //    order: the wait for the payload in step 4 then covers all of them)
         const uint32_t hdr2 = hdr_of(t2);
-        const uint64_t g2 = g0_of(t2);
+        const uint64_t w2 = t2 < ntot ? off_word(tile_off, t2) : 0ull;
+        const uint64_t g1 = g0_from(t1, w1);
         const bool fits1 = t1 < ntot && agg1 <= kMax && g1 + 4ull * agg1 <= it.s_len;
         const uint64_t a1 = g1 & ~15ull;
         const uint32_t nchunk1 = fits1 ? (((uint32_t)(g1 - a1) >> 2) + agg1 + 3) >> 2 : 0u;
@@ -1295,7 +1346,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
             __syncthreads();
         }
         t = t1; hdr = hdr1; g0 = g1; agg = agg1; off = off1; fits = fits1; t_start = t1_start;
-        t1 = t2; hdr1 = hdr2; g1 = g2;
+        t1 = t2; hdr1 = hdr2; w1 = w2;
     }
     finish_launch(ctrl, epoch);
 }
@@ -1334,7 +1385,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
     const TileQ tq = {0u, 1u};   // one counter: chunks are few
     if (tid == 0) {
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh.tile = tq.take(ctrl);
+        sh.tile = tq.id(tq.take(ctrl));
     }
     __syncthreads();
     const uint32_t epoch = sh.epoch;
@@ -1398,7 +1449,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
                 st_relaxed(tile_off + t, lb_word((epoch + 1u) & kEpochMask, LB_INCL, excl + incl - sh.tagg[lane]));
         }
         __syncthreads();
-        if (tid == 0) sh.tile = tq.take(ctrl);
+        if (tid == 0) sh.tile = tq.id(tq.take(ctrl));
         __syncthreads();
     }
     finish_launch(ctrl, epoch);
@@ -1439,6 +1490,12 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
         if (items)
             hipLaunchKernelGGL((compress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
                                cfg.total_tiles, cfg.d_total_tiles, ctrl, status, sc);
+        else if (single.dec.flags & DEC_LOSSLESS)
+            hipLaunchKernelGGL((compress_kernel<TILE, false, KIND_LOSSLESS>), grid, block, 0, stream, single, items,
+                               tile_items, cfg.total_tiles, cfg.d_total_tiles, ctrl, status, sc);
+        else if (single.dec.flags & DEC_Q24)
+            hipLaunchKernelGGL((compress_kernel<TILE, false, KIND_Q24>), grid, block, 0, stream, single, items,
+                               tile_items, cfg.total_tiles, cfg.d_total_tiles, ctrl, status, sc);
         else
             hipLaunchKernelGGL((compress_kernel<TILE, false>), grid, block, 0, stream, single, items, tile_items,
                                cfg.total_tiles, cfg.d_total_tiles, ctrl, status, sc);
@@ -1619,13 +1676,13 @@ __device__ __forceinline__ void serve_consume(const SrvArgs &a, Ctrl *ctrl, uint
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t win = sc.win;
     const TileQ q = {cls, sc.nc};
-    if (tid == 0) sh.next = q.take(ctrl);
+    if (tid == 0) sh.next = q.take(ctrl);   // raw (TileQ)
     __syncthreads();
     const auto item_in = [&](uint32_t buf) -> const Item & { return *(const Item *)ss.itm[buf]; };
 
     uint32_t np = 0, hb = 0;
     PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
-    uint32_t n = sh.next, par = 0;
+    uint32_t n = q.id(sh.next), par = 0;
     uint32_t late = 0;   // thread 0: rounds waited for n after the instance closed above it
     for (;; par ^= 1u) {
         // 0. is n published? (1: yes, 2: never in this instance, 0: not yet)
@@ -1682,8 +1739,8 @@ __device__ __forceinline__ void serve_consume(const SrvArgs &a, Ctrl *ctrl, uint
         const bool p_lb = resolve && p0.tl != 0;
         uint64_t w0[lb_per(TILE)];
         if (p_lb) lb_probe<TILE>(status, (int64_t)p0.tile - 1, item_in(p0.hb).first_tile, epoch, win, w0);
-        uint32_t next_id = n;
-        if (tid == 0 && have_n) next_id = q.take(ctrl);
+        uint32_t next_k = ~0u;   // raw (TileQ), written to LDS before the scan barrier
+        if (tid == 0 && have_n) next_k = q.take(ctrl);
 
         // 2. header of n; scan and the first half of p0's evaluation share a barrier
         uint32_t n_hdr = 0, n_words = 0;
@@ -1697,9 +1754,10 @@ __device__ __forceinline__ void serve_consume(const SrvArgs &a, Ctrl *ctrl, uint
         const uint32_t incl = wave_incl_scan(n_words, lane);
         if (lane == 63) sh.wsum[par][wave] = incl;
         if (p_lb) lb_slices<TILE>(w0, epoch, win, sh.lbf, par);
-        if (tid == 0) sh.next2[par] = next_id;
+        if (tid == 0) sh.next2[par] = next_k;
         __syncthreads();
-        const uint32_t n_next = sh.next2[par];
+        const uint32_t next_raw = sh.next2[par];
+        const uint32_t n_next = next_raw == ~0u ? n : q.id(next_raw);
         uint32_t before = 0, n_agg = 0;
 #pragma unroll
         for (int v = 0; v < TILE / 64; v++) {

@@ -79,7 +79,7 @@ EXPORTS = (
     "lscomp_decompress_u32", "lscomp_decompress_u32_async",
     "lscomp_container_bytes", "lscomp_container_pack", "lscomp_container_parse",
     "lscomp_set_launch_shape_ex", "lscomp_get_launch_shape_ex", "lscomp_server_create_device",
-    "lscomp_server_stats",
+    "lscomp_server_stats", "lscomp_plan_set_shape", "lscomp_plan_get_shape", "lscomp_plan_autotune",
 )
 
 REPLAY_HOLD = 1
@@ -108,8 +108,12 @@ class ErrorStats(ctypes.Structure):
 
 class TuneEntry(ctypes.Structure):
     """lscomp_tune_entry."""
-    _fields_ = [("tile", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("compress_ms", ctypes.c_float),
-                ("decompress_ms", ctypes.c_float)]
+    _fields_ = [("direction", ctypes.c_uint32), ("tile", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
+                ("lb_window", ctypes.c_uint32), ("counters", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {"direction": self.direction, "tile": self.tile, "wg_per_cu": self.wg_per_cu,
+                "lb_window": self.lb_window, "counters": self.counters, "ms": self.ms}
 
 
 class Request(ctypes.Structure):
@@ -189,6 +193,13 @@ def _load():
     for name in ("lscomp_plan_launch", "lscomp_plan_launch_graph", "lscomp_plan_status"):
         getattr(lib, name).restype = ctypes.c_int
         getattr(lib, name).argtypes = [vp, vp]
+    lib.lscomp_plan_set_shape.restype = ctypes.c_int
+    lib.lscomp_plan_set_shape.argtypes = [vp, ctypes.POINTER(LaunchShape), vp]
+    lib.lscomp_plan_get_shape.restype = ctypes.c_int
+    lib.lscomp_plan_get_shape.argtypes = [vp, ctypes.POINTER(LaunchShape)]
+    lib.lscomp_plan_autotune.restype = ctypes.c_int
+    lib.lscomp_plan_autotune.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(TuneEntry),
+                                         ctypes.POINTER(ctypes.c_uint32)]
     lib.lscomp_plan_device_sizes.restype = vp
     lib.lscomp_plan_device_sizes.argtypes = [vp]
     lib.lscomp_plan_sizes.restype = ctypes.c_int
@@ -520,15 +531,14 @@ def autotune(vol, dims, bins=LOSSLESS_BINS, pooling: float = 1.0, reps: int = 3,
     _check_volume(vol, dims)
     work = torch.empty(max_compressed_bytes(dims), dtype=torch.uint8, device=vol.device)
     dec = torch.empty_like(vol)
-    table = (TuneEntry * 32)()
-    n = ctypes.c_uint32(32)
+    table = (TuneEntry * 128)()
+    n = ctypes.c_uint32(128)
     st = _lib.lscomp_autotune_u16(ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(work.data_ptr()), work.numel(),
                                   ctypes.c_void_p(dec.data_ptr()), _dims(dims), _bins(bins), ctypes.c_float(pooling),
                                   reps, _stream_handle(stream), table, ctypes.byref(n))
     if st != OK:
         raise LscompError(st, "lscomp_autotune_u16")
-    return [{"tile": t.tile, "wg_per_cu": t.wg_per_cu, "compress_ms": t.compress_ms,
-             "decompress_ms": t.decompress_ms} for t in table[: n.value]]
+    return [t.as_dict() for t in table[: n.value]]
 
 
 def error_stats(orig, dec, stream=None) -> dict:
@@ -599,6 +609,29 @@ class Plan:
         if st != OK:
             raise LscompError(st, "lscomp_plan_set_items_u16")
         self.n = len(items)
+
+    def set_shape(self, tile: int = 0, wg_per_cu: int = 0, lb_window: int = 0, counters: int = 0,
+                  stream=None) -> None:
+        sh = LaunchShape(tile, wg_per_cu, lb_window, counters)
+        st = _lib.lscomp_plan_set_shape(self.h, ctypes.byref(sh), _stream_handle(stream))
+        if st != OK:
+            raise LscompError(st, "lscomp_plan_set_shape")
+
+    def get_shape(self) -> dict:
+        sh = LaunchShape()
+        st = _lib.lscomp_plan_get_shape(self.h, ctypes.byref(sh))
+        if st != OK:
+            raise LscompError(st, "lscomp_plan_get_shape")
+        return {"tile": sh.tile, "wg_per_cu": sh.wg_per_cu, "lb_window": sh.lb_window, "counters": sh.counters}
+
+    def autotune(self, reps: int = 3, stream=None):
+        """Tune the plan's launch shape on its current batch; returns the measured entries."""
+        table = (TuneEntry * 128)()
+        n = ctypes.c_uint32(128)
+        st = _lib.lscomp_plan_autotune(self.h, reps, _stream_handle(stream), table, ctypes.byref(n))
+        if st != OK:
+            raise LscompError(st, "lscomp_plan_autotune")
+        return [t.as_dict() for t in table[: n.value]]
 
     def launch(self, stream=None, graph: bool = False) -> None:
         f = _lib.lscomp_plan_launch_graph if graph else _lib.lscomp_plan_launch

@@ -83,3 +83,42 @@ def test_plan_rejects_bad_items(gpu):
     with pytest.raises(gpu.LscompError) as ei:      # more blocks than the plan holds
         plan.set_items([(big, (1, 1024, 1024), (1, 1, 1, 1), 1.0, o, 256)])
     plan.close()
+
+
+def test_plan_shape_change_and_autotune(gpu, orc):
+    """A plan's shape can change after its batch is set (tile tables rebuilt,
+    captured graph recaptured), and the plan autotuner, timing the plan's own
+    batch, applies one of the shapes it measured; every stream stays the
+    oracle's."""
+    import torch
+    vols = _frames(5, 20)
+    params = [((1, 1, 1, 1), 1.0) if i % 2 == 0 else ((3, 5, 10, 15), 0.5) for i in range(len(vols))]
+    dev = [torch.from_numpy(v.view(np.int16)).cuda() for v in vols]
+    outs = [torch.zeros(gpu.max_compressed_bytes(v.shape), dtype=torch.uint8, device="cuda") for v in vols]
+    refs = [orc.compress(v, b, p)[1] for v, (b, p) in zip(vols, params)]
+    plan = gpu.Plan(gpu.COMPRESS, len(vols), sum(gpu.num_blocks(v.shape) for v in vols))
+    plan.set_items([(d, v.shape, b, p, o, o.numel()) for d, v, (b, p), o in zip(dev, vols, params, outs)])
+
+    def check():
+        sizes = plan.sizes()
+        assert plan.status() == gpu.OK
+        for o, n, ref in zip(outs, sizes, refs):
+            assert n == ref.size and np.array_equal(o[:n].cpu().numpy(), ref)
+            o.zero_()
+
+    plan.launch(graph=True)
+    check()
+    for shape in ((128, 1, 128, 2), (256, 0, 64, 16), (512, 0, 256, 1)):
+        plan.set_shape(*shape)
+        assert plan.get_shape() == dict(zip(("tile", "wg_per_cu", "lb_window", "counters"), shape))
+        plan.launch(graph=True)
+        check()
+    table = plan.autotune(reps=1)
+    assert {e["tile"] for e in table} == {128, 256, 512} and all(e["direction"] == 0 for e in table)
+    cur = plan.get_shape()
+    assert any(all(e[f] == cur[f] for f in cur) for e in table)
+    for o in outs:
+        o.zero_()
+    plan.launch(graph=True)
+    check()
+    plan.close()

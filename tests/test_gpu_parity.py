@@ -190,21 +190,45 @@ def test_launch_shapes_give_identical_bytes(gpu, orc, tile, wg_per_cu):
 
 
 def test_autotune_applies_a_shape_and_keeps_bytes(gpu, orc):
+    """Coordinate descent over tile, workgroups per CU, window and counters for
+    both directions: every axis is measured, the applied shapes are ones it
+    measured, and the streams stay the oracle's."""
     import torch
     vol = synth("xpcs", (8, 512, 512), seed=17)
     d = to_dev(vol)
-    saved = [gpu.get_launch_shape(k) for k in (gpu.COMPRESS, gpu.DECOMPRESS)]
+    saved = [gpu.get_launch_shape_ex(k) for k in (gpu.COMPRESS, gpu.DECOMPRESS)]
     try:
         table = gpu.autotune(d, vol.shape, (1, 1, 1, 1), 1.0, reps=2)
-        assert len(table) == 15 and all(e["compress_ms"] > 0 and e["decompress_ms"] > 0 for e in table)
+        for k in (gpu.COMPRESS, gpu.DECOMPRESS):
+            mine = [e for e in table if e["direction"] == k]
+            assert all(e["ms"] > 0 for e in mine)
+            assert {e["tile"] for e in mine} == {128, 256, 512}
+            assert len({e["counters"] for e in mine}) == 5 and len({e["lb_window"] for e in mine}) >= 2
+            cur = gpu.get_launch_shape_ex(k)
+            assert any(all(e[f] == cur[f] for f in ("tile", "wg_per_cu", "lb_window", "counters")) for e in mine)
         assert gpu.autotune(d, vol.shape, (1, 1, 1, 1), 1.0) == []          # cached
-        tile, w = gpu.get_launch_shape(gpu.COMPRESS)
-        assert tile in (128, 256, 512)
         check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+        check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
     finally:
-        for k, (t, w) in zip((gpu.COMPRESS, gpu.DECOMPRESS), saved):
-            gpu.set_launch_shape(k, t, w)
+        for k, sh in zip((gpu.COMPRESS, gpu.DECOMPRESS), saved):
+            gpu.set_launch_shape_ex(k, **sh)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("shape", [(128, 0, 128, 1), (256, 1, 256, 16), (512, 2, 512, 8), (512, 0, 64, 3)])
+def test_every_window_and_counter_count_gives_identical_bytes(gpu, orc, shape):
+    """The stream does not depend on the look-back window or the number of tile-id counters."""
+    vol = synth("cssi", (5, 300, 1558), seed=19)
+    tile, w, win, nc = shape
+    saved = [gpu.get_launch_shape_ex(k) for k in (gpu.COMPRESS, gpu.DECOMPRESS)]
+    try:
+        for k in (gpu.COMPRESS, gpu.DECOMPRESS):
+            gpu.set_launch_shape_ex(k, tile, w, win, nc)
+        check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+        check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
+    finally:
+        for k, sh in zip((gpu.COMPRESS, gpu.DECOMPRESS), saved):
+            gpu.set_launch_shape_ex(k, **sh)
 
 
 @pytest.mark.parametrize("prepass", [0, 1, 2])
