@@ -76,6 +76,26 @@ int lscomp_compress_u16_async(const uint16_t *d_in, uint8_t *d_out, size_t out_c
                               uint64_t *d_out_bytes, lscomp_dim3 dims, lscomp_bins4 bins,
                               float pooling_th, hipStream_t stream);
 
+/* Size of the per-GPU device array behind lscomp_compress_u16_noalloc_async:
+ * 8 KiB of control block, then one 8-byte status word per tile. */
+#define LSCOMP_NOALLOC_SCRATCH_BYTES (4u << 20)
+
+/* lscomp_compress_u16_async that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, stream and result as lscomp_compress_u16_async; the
+ * look-back scratch is a fixed device array per GPU (the library's only
+ * allocation is that array, made when the library is loaded and held until
+ * the process ends).  Limit: ceil(blocks / tile) <= (4 MiB - 8 KiB) / 8 =
+ * 523264 tiles at the current compression tile (128, 256 or 512 blocks), else
+ * LSCOMP_ERR_TOO_LARGE.  Calls that overlap on one GPU (unordered streams)
+ * race on the array: order them.  lscomp_stream_status does not see this
+ * call's device error word; the only error it could hold for compression is
+ * a look-back time-out, an internal fault. */
+int lscomp_compress_u16_noalloc_async(const uint16_t *d_in, uint8_t *d_out, size_t out_capacity,
+                                      uint64_t *d_out_bytes, lscomp_dim3 dims, lscomp_bins4 bins,
+                                      float pooling_th, hipStream_t stream);
+
 /* Decompress; replaces cuLSZ_decompression_uint16_bsize64 (cuLSZ_entry.h:11).
  * Waits for completion and returns LSCOMP_ERR_CORRUPT if the stream is
  * inconsistent (those blocks decode to zeros).  Every element of the volume is

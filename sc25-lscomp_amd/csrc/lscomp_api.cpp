@@ -177,9 +177,6 @@ struct Scratch {
 std::mutex g_mu;
 std::map<std::pair<int, hipStream_t>, Scratch *> g_scratch;
 
-// Any period below 2^26 launches keeps stale words of an equal epoch out.
-constexpr uint64_t kEpochClearPeriod = 1ull << 25;
-
 Scratch *get_scratch(hipStream_t stream)
 {
     int dev = 0;
@@ -267,6 +264,10 @@ int take_errors(Scratch *s)
     return LSCOMP_ERR_CORRUPT;
 }
 
+}  // namespace
+
+namespace lsc {
+
 // Validates a compression call and fills its Item (no HIP calls).
 int prepare_compress(const uint16_t *d_in, uint8_t *d_out, size_t cap, lscomp_dim3 dims, lscomp_bins4 bins,
                      float p, uint32_t tile, Item &it)
@@ -295,6 +296,10 @@ int prepare_decompress(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, ls
     it.align = align_class(d_out, dims.z);
     return LSCOMP_OK;
 }
+
+}  // namespace lsc
+
+namespace {
 
 // uint32 volumes: validation and item (tiles of kTileU32 blocks; no HIP calls).
 int prepare_compress_u32(const uint32_t *d_in, uint8_t *d_out, size_t cap, lscomp_dim3 dims, lscomp_bins4 bins,

@@ -45,8 +45,6 @@ struct lscomp_plan_s {
 
 namespace {
 
-constexpr uint64_t kEpochClearPeriod = 1ull << 25;
-
 size_t table_bytes(uint32_t max_items, uint64_t max_tiles)
 {
     return sizeof(Item) * max_items + sizeof(uint32_t) * (max_tiles + 1);

@@ -38,5 +38,14 @@ int tune_shape(int direction, bool table, Shape start, const std::function<float
                Shape &best, std::vector<lscomp_tune_entry> &log);
 // Grid and scheduling of a launch of `total_tiles` tiles with a normalized shape.
 LaunchCfg make_cfg(uint64_t total_tiles, bool compress, bool table, const Shape &shape);
+// Validate a single-volume call and fill its Item (tiles of `tile` blocks; no
+// HIP calls; returns a status).
+int prepare_compress(const uint16_t *d_in, uint8_t *d_out, size_t cap, lscomp_dim3 dims, lscomp_bins4 bins,
+                     float p, uint32_t tile, Item &it);
+int prepare_decompress(uint16_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
+                       lscomp_bins4 bins, uint32_t tile, Item &it);
+// Look-back words carry a 26-bit epoch: cleared at least every 2^25 launches
+// that use them, no stale word of an equal epoch survives.
+constexpr uint64_t kEpochClearPeriod = 1ull << 25;
 
 }  // namespace lsc

@@ -121,8 +121,8 @@ struct DevLoop {
     uint64_t instances = 0;
 };
 
-// Wider than 2^25 instances the epoch tags of map and status words could repeat.
-constexpr uint64_t kEpochClearPeriod = 1ull << 25;
+// Map and status words are cleared every kEpochClearPeriod instances: their
+// epoch tags cannot repeat in between.
 
 uint64_t load_u64(const uint64_t *p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
 uint32_t load_u32(const uint32_t *p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
@@ -322,7 +322,7 @@ int dev_launch(lscomp_server s)
     if (d.running || load_u64(&d.hc->rd) >= d.wr) return LSCOMP_OK;
     // a new instance: clear its close word (and, rarely, the tagged words)
     int st = LSCOMP_OK;
-    if (++d.instances % kEpochClearPeriod == 0) {
+    if (++d.instances % lsc::kEpochClearPeriod == 0) {
         st = lsc::hip_fail(lsc::launch_fill_u64(d.args.map, lsc::kSrvMaxTiles, 0, d.stream), "fill");
         if (!st) st = lsc::hip_fail(lsc::launch_fill_u64(d.status, lsc::kSrvMaxTiles, 0, d.stream), "fill");
     }

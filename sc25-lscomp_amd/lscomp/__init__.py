@@ -67,7 +67,7 @@ class ItemDesc(ctypes.Structure):
 # Every symbol include/lscomp.h declares.
 EXPORTS = (
     "lscomp_num_blocks_u16", "lscomp_header_bytes_u16", "lscomp_max_compressed_bytes_u16",
-    "lscomp_compress_u16", "lscomp_compress_u16_async", "lscomp_decompress_u16",
+    "lscomp_compress_u16", "lscomp_compress_u16_async", "lscomp_compress_u16_noalloc_async", "lscomp_decompress_u16",
     "lscomp_decompress_u16_async", "lscomp_stream_status", "lscomp_reserve_u16",
     "lscomp_release_all", "lscomp_set_launch_shape", "lscomp_get_launch_shape", "lscomp_status_string",
     "lscomp_version", "lscomp_plan_create", "lscomp_plan_set_items_u16", "lscomp_plan_launch",
@@ -167,6 +167,8 @@ def _load():
     lib.lscomp_compress_u16.argtypes = [vp, vp, sz, ctypes.POINTER(sz), Dim3, Bins4, ctypes.c_float, vp]
     lib.lscomp_compress_u16_async.restype = ctypes.c_int
     lib.lscomp_compress_u16_async.argtypes = [vp, vp, sz, vp, Dim3, Bins4, ctypes.c_float, vp]
+    lib.lscomp_compress_u16_noalloc_async.restype = ctypes.c_int
+    lib.lscomp_compress_u16_noalloc_async.argtypes = [vp, vp, sz, vp, Dim3, Bins4, ctypes.c_float, vp]
     lib.lscomp_decompress_u16.restype = ctypes.c_int
     lib.lscomp_decompress_u16.argtypes = [vp, vp, sz, Dim3, Bins4, vp]
     lib.lscomp_decompress_u16_async.restype = ctypes.c_int
@@ -318,6 +320,18 @@ def compress_async(vol, dims, bins, pooling, out, d_size, stream=None) -> None:
                                         _bins(bins), ctypes.c_float(pooling), _stream_handle(stream))
     if st != OK:
         raise LscompError(st, "lscomp_compress_u16_async")
+
+
+def compress_noalloc_async(vol, dims, bins, pooling, out, d_size, stream=None) -> None:
+    """compress_async through the entry point that allocates nothing (a fixed
+    4 MiB device array per GPU holds its scratch; calls that overlap on one GPU
+    race on it)."""
+    _check_volume(vol, dims)
+    st = _lib.lscomp_compress_u16_noalloc_async(ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                                out.numel(), ctypes.c_void_p(d_size.data_ptr()), _dims(dims),
+                                                _bins(bins), ctypes.c_float(pooling), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_compress_u16_noalloc_async")
 
 
 def decompress(stream_buf, nbytes: int, dims, bins=LOSSLESS_BINS, out=None, stream=None):
