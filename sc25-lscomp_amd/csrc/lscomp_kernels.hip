@@ -10,22 +10,42 @@
 //   3. workgroup scan of payload sizes, tile aggregate published at once;
 //   4. a tile stays pending, packed in LDS, for two iterations while the
 //      workgroup loads and packs the next ones; its look-back runs then;
-//   5. decoupled look-back with a window of 128 predecessors per probe round,
-//      one 64-bit word per tile (epoch | flag | value) read and written with
-//      relaxed agent-scope atomics, so no fences are needed on the multi-XCD
-//      part;
+//   5. decoupled look-back with a window of `win` (a launch parameter, 64 by
+//      default) predecessors per probe round, one 64-bit word per tile
+//      (epoch | flag | value) read and written with relaxed agent-scope
+//      atomics, so no fences are needed on the multi-XCD part;
 //   6. headers and payload leave LDS as contiguous coalesced 16-byte stores.
 // Decompression (replaces cuLSZ_decompression_kernel_uint16_bsize64,
-// cuLSZ_kernel.cu:1286-1646) takes tile offsets from a pre-pass over the
-// header bytes, stages each tile's payload in LDS with coalesced loads (for
-// dense streams one tile ahead), and each lane unpacks its block and writes
-// its rows.  Rate-0 blocks decode to zeros.  uint32 volumes run the same
+// cuLSZ_kernel.cu:1286-1646) takes tile offsets from a first phase over the
+// header bytes (or a pre-pass kernel), stages each tile's payload in LDS with
+// coalesced loads (for dense streams one tile ahead), and each lane unpacks
+// its block and writes its rows.  Rate-0 blocks decode to zeros.  uint32 volumes run the same
 // kernels with the uint32 codec.
 #include "lscomp_internal.h"
 
 namespace lsc {
 
 constexpr uint32_t kSpinLimit = 1u << 16;
+
+// 16 bytes at 4-byte alignment.  Volume rows and stream bytes are in general
+// only dword aligned (volumes and stream buffers at 4-byte alignment are
+// accepted, and stream offsets are multiples of 16 from the buffer's start);
+// gfx950's global_load_dwordx4 / global_store_dwordx4 take that.  This type
+// states the alignment these accesses rely on (a uint4 pointer would promise
+// 16).  LDS accesses stay uint4: their addresses are 16-byte aligned.
+typedef uint32_t u32x4_a4 __attribute__((__vector_size__(16), __aligned__(4)));
+
+__device__ __forceinline__ uint4 ld16_a4(const void *p)
+{
+    const u32x4_a4 v = *(const u32x4_a4 *)p;
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+__device__ __forceinline__ void st16_a4(void *p, uint4 x)
+{
+    const u32x4_a4 v = {x.x, x.y, x.z, x.w};
+    *(u32x4_a4 *)p = v;
+}
 
 __device__ __forceinline__ uint64_t lb_word(uint32_t epoch, uint32_t flag, uint64_t val)
 {
@@ -96,7 +116,7 @@ __device__ __forceinline__ void load_block(const Item &it, uint32_t bx, uint32_t
             t[r] = make_uint4(0, 0, 0, 0);
             if ((uint32_t)r < ny) {
                 if (nz == 8u || q + 8 <= vol_end) {
-                    t[r] = *(const uint4 *)q;
+                    t[r] = ld16_a4(q);
                 } else {
                     const uint32_t *q32 = (const uint32_t *)q;
                     t[r].x = q32[0];   // keep >= 1 (nz is even and nonzero)
@@ -128,7 +148,7 @@ __device__ __forceinline__ void load_block(const Item &it, uint32_t bx, uint32_t
             bool got = false;
             uint32_t w[4];
             if (rv && ((uintptr_t)q & 2u) == 0 && q + 8 <= vol_end) {
-                const uint4 t = *(const uint4 *)q;
+                const uint4 t = ld16_a4(q);
                 w[0] = t.x;
                 w[1] = t.y;
                 w[2] = t.z;
@@ -136,7 +156,7 @@ __device__ __forceinline__ void load_block(const Item &it, uint32_t bx, uint32_t
                 got = true;
             } else if (rv && q > it.vol_in && q + 9 <= vol_end) {
                 const uint32_t *a = (const uint32_t *)(q - 1);   // dword boundary
-                const uint4 t = *(const uint4 *)a;
+                const uint4 t = ld16_a4(a);
                 const uint32_t e = a[4];
                 w[0] = __builtin_amdgcn_alignbit(t.y, t.x, 16);
                 w[1] = __builtin_amdgcn_alignbit(t.z, t.y, 16);
@@ -168,20 +188,34 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
     const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
     const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
     if (it.align != ALIGN2) {
-        // 16-byte row stores (dword alignment suffices, as for the loads); a
-        // block at the end of a row stores its nz/2 dwords
+        // A whole row stores its first 12 bytes, a block at the end of a row
+        // its first nz/2 - 1 dwords; both then store their last dword with one
+        // shared instruction.  Timed on the CSSI volume, this was 1.5-2.5 %
+        // faster than one 16-byte store per row (lossless decode 0.829 ->
+        // 0.819 ms, lossy 0.748 -> 0.729), and 4 % faster than 8 16-byte
+        // stores with no per-row branch for whole blocks.
+        typedef uint32_t u32x3_a4 __attribute__((ext_vector_type(3), __aligned__(4)));
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             if ((uint32_t)r >= ny) break;
-            uint16_t *q = p + (uint64_t)r * it.z;
+            uint32_t *q32 = (uint32_t *)(p + (uint64_t)r * it.z);   // dword aligned
+            const uint32_t d0 = D[4 * r], d1 = D[4 * r + 1], d2 = D[4 * r + 2], d3 = D[4 * r + 3];
+            uint32_t tail, tail_at;   // the row's last dword and its index
+            bool has_tail;
             if (nz == 8u) {
-                *(uint4 *)q = make_uint4(D[4 * r], D[4 * r + 1], D[4 * r + 2], D[4 * r + 3]);
+                const u32x3_a4 t = {d0, d1, d2};
+                *(u32x3_a4 *)q32 = t;
+                tail = d3;
+                tail_at = 3;
+                has_tail = true;
             } else {
-                uint32_t *q32 = (uint32_t *)q;
-                q32[0] = D[4 * r];
-                if (nz > 2u) q32[1] = D[4 * r + 1];
-                if (nz > 4u) q32[2] = D[4 * r + 2];
+                q32[0] = d0;
+                if (nz > 2u) q32[1] = d1;
+                tail = d2;
+                tail_at = 2;
+                has_tail = nz > 4u;
             }
+            if (has_tail) q32[tail_at] = tail;
         }
     } else {
         // Rows only 2-byte aligned: a row at a 4-byte boundary stores 16 bytes, a
@@ -194,7 +228,7 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
             uint16_t *q = p + (uint64_t)r * it.z;
             const uint32_t d0 = D[4 * r], d1 = D[4 * r + 1], d2 = D[4 * r + 2], d3 = D[4 * r + 3];
             if (nz == 8u && ((uintptr_t)q & 2u) == 0) {
-                *(uint4 *)q = make_uint4(d0, d1, d2, d3);
+                st16_a4(q, make_uint4(d0, d1, d2, d3));
             } else if (nz == 8u) {
                 q[0] = (uint16_t)(d0 & 0xffffu);
                 uint32_t *m = (uint32_t *)(q + 1);   // dword boundary
@@ -248,8 +282,8 @@ __device__ __forceinline__ void load_block_u32(const Item &it, uint32_t bx, uint
         t[2 * r + 1] = make_uint4(0, 0, 0, 0);
         if ((uint32_t)r < ny) {
             if (nz == 8u || q + 8 <= vol_end) {
-                t[2 * r] = *(const uint4 *)q;
-                t[2 * r + 1] = *(const uint4 *)(q + 4);
+                t[2 * r] = ld16_a4(q);
+                t[2 * r + 1] = ld16_a4(q + 4);
             } else {
                 uint32_t e[8];
 #pragma unroll
@@ -279,9 +313,8 @@ __device__ __forceinline__ void store_block_u32(const Item &it, uint32_t bx, uin
     if (ny == 8u && nz == 8u) {
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            *(uint4 *)(p + (uint64_t)r * it.z) = make_uint4(V[8 * r], V[8 * r + 1], V[8 * r + 2], V[8 * r + 3]);
-            *(uint4 *)(p + (uint64_t)r * it.z + 4) =
-                make_uint4(V[8 * r + 4], V[8 * r + 5], V[8 * r + 6], V[8 * r + 7]);
+            st16_a4(p + (uint64_t)r * it.z, make_uint4(V[8 * r], V[8 * r + 1], V[8 * r + 2], V[8 * r + 3]));
+            st16_a4(p + (uint64_t)r * it.z + 4, make_uint4(V[8 * r + 4], V[8 * r + 5], V[8 * r + 6], V[8 * r + 7]));
         }
     } else {   // by elements (a variant storing row ends as 16-, 8- and 4-byte pieces decoded 2.3x slower)
 #pragma unroll
@@ -408,7 +441,7 @@ __device__ __forceinline__ void store_payload(const uint32_t *pay, uint32_t word
 __device__ __forceinline__ uint4 load_chunk(const uint8_t *s_in, uint64_t s_len, uint64_t a0, uint32_t c)
 {
     const uint64_t at = a0 + 16ull * c;
-    if (at + 16 <= s_len) return *(const uint4 *)(s_in + at);
+    if (at + 16 <= s_len) return ld16_a4(s_in + at);
     const uint32_t *s32 = (const uint32_t *)(s_in + at);
     uint32_t t[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -754,7 +787,7 @@ __device__ __forceinline__ void store_tile(const Item &it, uint32_t tl, uint32_t
     if (tid < (uint32_t)TILE / 16) {
         const uint64_t o = (uint64_t)tl * TILE + 16ull * tid;
         if (o + 16 <= it.hdr_bytes) {
-            *(uint4 *)(it.s_out + o) = make_uint4(hdr[4 * tid], hdr[4 * tid + 1], hdr[4 * tid + 2], hdr[4 * tid + 3]);
+            st16_a4(it.s_out + o, make_uint4(hdr[4 * tid], hdr[4 * tid + 1], hdr[4 * tid + 2], hdr[4 * tid + 3]));
         } else {
 #pragma unroll
             for (uint32_t k = 0; k < 4; k++)
@@ -960,7 +993,7 @@ __device__ __forceinline__ uint32_t words16(const Item &it, uint64_t off)
     if (off >= it.nblocks) return 0;
     uint4 x;
     if (off + 16 <= it.nblocks) {
-        x = *(const uint4 *)(it.s_in + off);
+        x = ld16_a4(it.s_in + off);
     } else {
         const uint32_t *s32 = (const uint32_t *)(it.s_in + off);
         uint32_t t[4] = {0, 0, 0, 0};
@@ -1406,7 +1439,7 @@ __global__ __launch_bounds__(256) void offsets_kernel(const Item it, Ctrl *ctrl,
             if (off >= it.nblocks) break;
             uint4 x;
             if (off + 16 <= it.nblocks) {
-                x = *(const uint4 *)(it.s_in + off);
+                x = ld16_a4(it.s_in + off);
             } else {   // the section's last piece: dwords below R, bytes past the last block cleared
                 const uint32_t *s32 = (const uint32_t *)(it.s_in + off);
                 uint32_t t[4] = {0, 0, 0, 0};
