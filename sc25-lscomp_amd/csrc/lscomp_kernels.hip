@@ -728,12 +728,31 @@ __device__ __forceinline__ TileQ uniform_queue(uint32_t cls, Sched sc)
     return TileQ{(uint32_t)__builtin_amdgcn_readfirstlane(cls), (uint32_t)__builtin_amdgcn_readfirstlane(sc.nc)};
 }
 
+// The single item, read again from the kernel argument segment where it is
+// used.  Kernels that take their Item by value otherwise load all of its
+// fields at entry and keep them live in scalar registers for the whole
+// persistent loop; with the loop's own state that spilled 60-105 scalar
+// registers into vector lanes (hundreds of v_readlane per tile).  Read
+// through a pointer the compiler cannot hoist (the empty asm), the fields are
+// scalar loads from the argument segment, near the use (lossy CSSI
+// compression 0.779 -> 0.751 ms, uint32 1.59 -> 1.41 ms).  Only for kernels
+// whose first parameter is `const Item`: the first explicit argument sits at
+// offset 0 of the segment.
+typedef const __attribute__((address_space(4))) Item const_item;
+__device__ __forceinline__ const Item &arg_item()
+{
+    const const_item *p = (const const_item *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const Item *)p;
+}
+
 // Item of a global tile: the single item, or items[tile_item[tile]].
 template <bool kTable>
 __device__ __forceinline__ const Item &tile_item(const Item &single, const Item *items, const uint32_t *tile_item,
                                                  uint32_t tile)
 {
-    if (!kTable) return single;
+    (void)single;
+    if (!kTable) return arg_item();
     return items[__builtin_amdgcn_readfirstlane(tile_item[tile])];
 }
 
@@ -743,7 +762,8 @@ __device__ __forceinline__ const Item &tile_item(const Item &single, const Item 
 template <bool kTable>
 __device__ __forceinline__ const Item &item_at(const Item &single, const Item *items, uint32_t idx)
 {
-    if (!kTable) return single;
+    (void)single;
+    if (!kTable) return arg_item();
     return items[idx];
 }
 
@@ -1247,9 +1267,10 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // payload is a few hundred bytes, the decode with the payload loaded in its own
 // iteration was 2-5 % faster in timings.
 template <class C, int TILE>
-__device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl, uint64_t *status,
+__device__ __forceinline__ void decompress_pipe_tiles(const Item it_arg, Ctrl *ctrl, uint64_t *status,
                                                       uint64_t *__restrict__ tile_off, Sched sc, uint32_t phase)
 {
+    const Item &it = it_arg;
     constexpr uint32_t kMax = (uint32_t)TILE * C::kMaxWords;   // payload words of a tile at most
     constexpr uint32_t kRing = kMax + 8u;                     // staging words (sh.pay)
     __shared__ TileShared<TILE, C::kMaxWords> sh;
@@ -1272,22 +1293,22 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
     uint32_t t = q.id(sh.tile), t1 = q.id(sh.next);
 
     // header byte of this thread's block in tile u, and u's payload byte offset
-    const auto hdr_of = [&](uint32_t u) -> uint32_t {
+    const auto hdr_of = [&](const Item &it, uint32_t u) -> uint32_t {
         const uint32_t b = u * (uint32_t)TILE + tid;
         return (u < ntot && b < it.nblocks) ? (uint32_t)it.s_in[b] : 0u;
     };
-    const auto g0_of = [&](uint32_t u) -> uint64_t {
+    const auto g0_of = [&](const Item &it, uint32_t u) -> uint64_t {
         return u < ntot ? (uint64_t)it.hdr_bytes + 4ull * tile_offset(tile_off, u, epoch, ctrl) : 0ull;
     };
     // from a word loaded earlier (off_word), checked now
-    const auto g0_from = [&](uint32_t u, uint64_t w) -> uint64_t {
+    const auto g0_from = [&](const Item &it, uint32_t u, uint64_t w) -> uint64_t {
         return u < ntot ? (uint64_t)it.hdr_bytes + 4ull * off_value(tile_off, u, w, epoch, ctrl) : 0ull;
     };
 
     // tile t: scan, then its payload staged at the start of the ring; t1's
     // offset word is checked where its payload loads need it
-    uint32_t hdr = hdr_of(t), hdr1 = hdr_of(t1);
-    uint64_t g0 = g0_of(t), w1 = t1 < ntot ? off_word(tile_off, t1) : 0ull;
+    uint32_t hdr = hdr_of(it, t), hdr1 = hdr_of(it, t1);
+    uint64_t g0 = g0_of(it, t), w1 = t1 < ntot ? off_word(tile_off, t1) : 0ull;
     uint32_t agg = 0, t_start = 0;
     uint32_t off = tile_scan<TILE>(header_payload_bytes(hdr) >> 2, tid, sh.wsum[0], &agg);
     bool fits = agg <= kMax && g0 + 4ull * agg <= it.s_len;
@@ -1295,6 +1316,7 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
     __syncthreads();
 
     while (t < ntot) {
+        const Item &it = arg_item();   // read near its uses (arg_item); uint32 decode 1.52 -> 1.47 ms
         // 1. scan of t1's headers; t2's id
         const uint32_t words1 = header_payload_bytes(hdr1) >> 2;
         const uint32_t incl = wave_incl_scan(words1, lane);
@@ -1314,9 +1336,9 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it, Ctrl *ctrl,
 
         // 2. t2's header byte and offset word, t1's payload loads (in this
         //    order: the wait for the payload in step 4 then covers all of them)
-        const uint32_t hdr2 = hdr_of(t2);
+        const uint32_t hdr2 = hdr_of(it, t2);
         const uint64_t w2 = t2 < ntot ? off_word(tile_off, t2) : 0ull;
-        const uint64_t g1 = g0_from(t1, w1);
+        const uint64_t g1 = g0_from(it, t1, w1);
         const bool fits1 = t1 < ntot && agg1 <= kMax && g1 + 4ull * agg1 <= it.s_len;
         const uint64_t a1 = g1 & ~15ull;
         const uint32_t nchunk1 = fits1 ? (((uint32_t)(g1 - a1) >> 2) + agg1 + 3) >> 2 : 0u;

@@ -77,6 +77,12 @@ def test_argument_validation_without_gpu():
     assert comp(dout=ctypes.c_void_p(0x20002)) == E   # stream must be 4-byte aligned
     assert comp(c=3) == lscomp.ERR_CAPACITY and n.value == 4
     assert comp(dims=(65536, 65536, 65536)) == lscomp.ERR_TOO_LARGE
+    # y or z near 2^32: the block counts are formed in 64 bits (a 32-bit ceil
+    # wrapped to 0 and divided by zero on the host); validation goes on to the
+    # null pointer
+    for dims in ((1, 0xFFFFFFFF, 8), (1, 8, 0xFFFFFFFF), (1, 0xFFFFFFF9, 16)):
+        assert comp(dims=dims, din=ctypes.c_void_p(0)) == E, dims
+        assert L.lscomp_num_blocks_u16(D(*dims)) == dims[0] * ((dims[1] + 7) // 8) * ((dims[2] + 7) // 8)
     assert L.lscomp_compress_u16_async(ok_in, ok_out, cap, None, D(1, 8, 8), B(1, 1, 1, 1),
                                        ctypes.c_float(1.0), None) == E
     dec = L.lscomp_decompress_u16

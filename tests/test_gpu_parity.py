@@ -231,6 +231,47 @@ def test_every_window_and_counter_count_gives_identical_bytes(gpu, orc, shape):
             gpu.set_launch_shape_ex(k, **sh)
 
 
+def test_shape_changes_on_another_thread(gpu, orc):
+    """A call reads the launch shape once (its tile count, scratch and grid all
+    use that copy), so a shape changed by another thread mid-call (as the
+    autotuner does) never leaves part of a volume unwritten."""
+    import threading
+    import torch
+    vol = synth("cssi", (7, 200, 1558), seed=23)
+    dims = vol.shape
+    refs = {}
+    for bins, p in ((1, 1, 1, 1), 1.0), ((3, 5, 10, 15), 0.5):
+        st, ref, refn = orc.compress(vol, bins, p)
+        refs[bins] = (p, ref, orc.decompress(ref, dims, bins)[1])
+    d_in = torch.from_numpy(vol.view(np.int16)).cuda()
+    saved = [gpu.get_launch_shape_ex(k) for k in (gpu.COMPRESS, gpu.DECOMPRESS)]
+    stop = threading.Event()
+
+    def flip():
+        shapes = [(128, 0, 128, 2), (512, 0, 64, 4), (256, 1, 256, 1), (512, 2, 512, 16)]
+        i = 0
+        while not stop.is_set():
+            for k in (gpu.COMPRESS, gpu.DECOMPRESS):
+                gpu.set_launch_shape_ex(k, *shapes[i % len(shapes)])
+            i += 1
+
+    t = threading.Thread(target=flip)
+    t.start()
+    try:
+        for it in range(24):
+            bins = (1, 1, 1, 1) if it % 2 == 0 else (3, 5, 10, 15)
+            p, ref, refdec = refs[bins]
+            out, n = gpu.compress(d_in, dims, bins, p)
+            assert n == ref.size and np.array_equal(out[:n].cpu().numpy(), ref), it
+            dec = gpu.decompress(out, n, dims, bins).cpu().numpy().view(np.uint16)
+            assert np.array_equal(dec, refdec), it
+    finally:
+        stop.set()
+        t.join()
+        for k, sh in zip((gpu.COMPRESS, gpu.DECOMPRESS), saved):
+            gpu.set_launch_shape_ex(k, **sh)
+
+
 @pytest.mark.parametrize("prepass", [0, 1, 2])
 def test_decode_with_and_without_prepass(gpu, orc, prepass):
     """Decode tile offsets from the in-kernel look-back (0), the offset phase of
