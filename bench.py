@@ -158,7 +158,7 @@ def cpu_info():
     return model
 
 
-def cpu_baseline(vol, bins, pooling, frames_all=48, frames_one=8, reps=5):
+def cpu_baseline(vol, bins, pooling, frames_all=600, frames_one=96, reps=5):
     """The oracle (C; OpenMP two-pass variant on all threads, serial variant on
     one) on the first frames of the volume: one warm-up, then the median of
     `reps` passes of the same in-memory buffer."""
@@ -395,7 +395,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=CSSI_DIMS[0], help="frames of the CSSI volume (default 600)")
-    ap.add_argument("--cpu-frames", type=int, default=48, help="frames of the CPU baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=CSSI_DIMS[0],
+                    help="frames of the CPU baseline's threaded sample (default: the whole volume)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--modes", default="lossless,lossy,xpcs,small,u32",
@@ -474,7 +475,8 @@ def main():
     achieved = (in_bytes + s_bytes) / (ll["compress_kernel_ms"] * 1e-3)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(vol, (1, 1, 1, 1), 1.0, frames_all=min(args.cpu_frames, dims[0]))
+        cpu = cpu_baseline(vol, (1, 1, 1, 1), 1.0, frames_all=min(args.cpu_frames, dims[0]),
+                           frames_one=min(96, dims[0]))
         log(f"cpu: {json.dumps(cpu)}")
     rec = {
         "metric": "compress/decompress GB/s + % of HBM peak on 1 MI355X; p99 per-frame latency",
