@@ -462,7 +462,7 @@ __device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len
     const uint64_t a0 = g0 & ~15ull;
     const uint32_t shift = (uint32_t)(g0 - a0) >> 2;
     const uint32_t nchunk = (shift + words + 3) >> 2;
-    for (uint32_t base = first_chunk; base < nchunk; base += 4u * TILE) {
+    const auto pass = [&](uint32_t base) {
         uint4 v[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {      // up to 4 loads in flight per lane
@@ -474,7 +474,11 @@ __device__ __forceinline__ void load_payload(const uint8_t *s_in, uint64_t s_len
             const uint32_t c = base + (uint32_t)k * TILE + tid;
             if (c < nchunk) *(uint4 *)(pay + 4 * c) = v[k];
         }
-    }
+    };
+    // the first pass outside the loop: a loop header before it made the
+    // loads wait for every store in flight
+    if (first_chunk < nchunk) pass(first_chunk);
+    for (uint32_t base = first_chunk + 4u * TILE; base < nchunk; base += 4u * TILE) pass(base);
 }
 
 // Look-back window `win` (a launch parameter, a multiple of 64 up to the
@@ -1139,7 +1143,10 @@ __device__ __forceinline__ void offset_phase(const Item &it, Ctrl *ctrl, uint64_
     }
 }
 
-template <int TILE, bool kTable>
+// kOff: the tile offsets are precomputed in tile_off (by the launch's offset
+// phase or the pre-pass kernel); otherwise each tile finds its offset with a
+// look-back (batches, and lscomp_set_decode_prepass(0)).
+template <int TILE, bool kTable, bool kOff = false>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_kernel(const Item single, const Item *__restrict__ items,
                                                           const uint32_t *__restrict__ tile_items,
                                                           uint32_t total_tiles, const uint32_t *d_total_tiles,
@@ -1159,7 +1166,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         sh.cls = tile_class(ctrl, sc);
         const TileQ q0{sh.cls, sc.nc};
         sh.tile = q0.id(q0.take(ctrl));
-        if (tile_off) sh.next = q0.take(ctrl);   // raw (TileQ)
+        if (kOff) sh.next = q0.take(ctrl);   // raw (TileQ)
     }
     __syncthreads();
     const TileQ q = uniform_queue(sh.cls, sc);
@@ -1170,35 +1177,41 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     // iterations later, so the atomic's round trip never stalls a wave); that
     // is safe because no tile waits for another.  With the look-back, ids are
     // taken one at a time.
+    // The next tile's header byte crosses the iteration in LDS (this thread's
+    // word of sh.hdr, unused by the decode): a register carried around the
+    // loop is copied at the loop's end, and that copy waited for every store
+    // of the iteration.
+    uint32_t *const hdr_slot = &sh.hdr[0][0] + tid;
     uint32_t hdr = 0;
     uint64_t w_next = 0;   // offset word of the next tile, checked when used
-    if (tile_off && sh.tile < ntot) {
+    if (kOff && sh.tile < ntot) {
         const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
         const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
         hdr = b < it.nblocks ? (uint32_t)it.s_in[b] : 0u;
         w_next = off_word(tile_off, sh.tile);
     }
+    if (kOff) *hdr_slot = hdr;
 
     for (;;) {
         const uint32_t tile = sh.tile;
         if (tile >= ntot) break;
-        const uint32_t ntile = tile_off ? q.id(sh.next) : ntot;
+        const uint32_t ntile = kOff ? q.id(sh.next) : ntot;
         uint32_t after_next = 0;   // thread 0: the raw id two tiles ahead
-        if (tile_off && tid == 0) after_next = q.take(ctrl);
+        if (kOff && tid == 0) after_next = q.take(ctrl);
         const Item &it = tile_item<kTable>(single, items, tile_items, tile);
         const uint32_t tl = tile - it.first_tile;
         const uint32_t b = tl * (uint32_t)TILE + tid;
         const bool valid = b < it.nblocks;
-        if (!tile_off) hdr = valid ? (uint32_t)it.s_in[b] : 0u;
+        hdr = kOff ? *hdr_slot : valid ? (uint32_t)it.s_in[b] : 0u;
 
         // with precomputed tile offsets no look-back is needed
-        uint64_t excl = tile_off ? off_value(tile_off, tile, w_next, epoch, ctrl) : 0ull;
+        uint64_t excl = kOff ? off_value(tile_off, tile, w_next, epoch, ctrl) : 0ull;
         const uint32_t words = header_payload_bytes(hdr) >> 2;
         bool bad = (hdr & 0x1fu) > 16u;
 
         uint32_t agg;
         const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum[0], &agg);
-        if (!tile_off) {
+        if (!kOff) {
             if (tid == 0) st_relaxed(status + tile, lb_word(epoch, tl == 0 ? LB_INCL : LB_AGG, agg));
             excl = 0;
             if (tl != 0) {
@@ -1210,15 +1223,21 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         const bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
         // the tile's payload lands in LDS at word `shift` (g0 mod 16, in words)
         const uint32_t shift = (uint32_t)(g0 & 15) >> 2;
-        if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
-        // next tile's header byte and offset
+        // next tile's header byte and offset, issued before this tile's payload
+        // loads and waited for with them, before this tile's stores: a wait for
+        // a load issued after the stores would wait for the stores too (one
+        // in-order counter for loads and stores)
         const uint32_t hdr_cur = hdr;
+        uint32_t hdr_next = 0;
         if (ntile < ntot) {
             const Item &nit = tile_item<kTable>(single, items, tile_items, ntile);
             const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
-            hdr = nb < nit.nblocks ? (uint32_t)nit.s_in[nb] : 0u;
-            if (tile_off) w_next = off_word(tile_off, ntile);
+            hdr_next = nb < nit.nblocks ? (uint32_t)nit.s_in[nb] : 0u;
+            if (kOff) w_next = off_word(tile_off, ntile);
         }
+        if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
+        if (kOff) *hdr_slot = hdr_next;   // waits for the byte here, before this tile's stores
+        if (kOff && tid == 0) sh.next = after_next;   // every thread read sh.next before the scan's barrier
         __syncthreads();
 
         if (valid) {
@@ -1238,9 +1257,8 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         }
         __syncthreads();
         if (tid == 0) {
-            if (tile_off) {
+            if (kOff) {
                 sh.tile = ntile;
-                sh.next = after_next;
             } else {
                 sh.tile = q.id(q.take(ctrl));   // no tile is taken ahead here
             }
@@ -1565,6 +1583,10 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
             if (cfg.tile_off && use_pipe_decode(single))
                 hipLaunchKernelGGL((decompress_pipe_kernel<TILE>), grid, block, 0, stream, single, ctrl, status,
                                    cfg.tile_off, sc, phase);
+            else if (cfg.tile_off)
+                hipLaunchKernelGGL((decompress_kernel<TILE, false, true>), grid, block, 0, stream, single, items,
+                                   tile_items, cfg.total_tiles, cfg.d_total_tiles, ctrl, status, cfg.tile_off, sc,
+                                   phase);
             else
                 hipLaunchKernelGGL((decompress_kernel<TILE, false>), grid, block, 0, stream, single, items,
                                    tile_items, cfg.total_tiles, cfg.d_total_tiles, ctrl, status, cfg.tile_off, sc,
