@@ -110,27 +110,46 @@ __device__ __forceinline__ void load_block(const Item &it, uint32_t bx, uint32_t
         const uint16_t *vol_end = it.vol_in + (uint64_t)it.x * it.y * it.z;
         const uint32_t keep = nz >> 1;   // dwords of a row inside the volume
         uint4 t[8];
+        if (ny == 8u && p + 7ull * it.z + 8 <= vol_end) {
+            // every row's 16 bytes lie inside the volume (almost every block):
+            // eight loads with no per-row tests (per-row tests cost about 15
+            // scalar and vector instructions a row; lossy CSSI compression
+            // 0.752 -> 0.738 ms without them, 0.729 with the whole-row copy below)
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const uint16_t *q = p + (uint64_t)r * it.z;
-            t[r] = make_uint4(0, 0, 0, 0);
-            if ((uint32_t)r < ny) {
-                if (nz == 8u || q + 8 <= vol_end) {
-                    t[r] = ld16_a4(q);
-                } else {
-                    const uint32_t *q32 = (const uint32_t *)q;
-                    t[r].x = q32[0];   // keep >= 1 (nz is even and nonzero)
-                    if (keep > 1) t[r].y = q32[1];
-                    if (keep > 2) t[r].z = q32[2];
+            for (int r = 0; r < 8; r++) t[r] = ld16_a4(p + (uint64_t)r * it.z);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const uint16_t *q = p + (uint64_t)r * it.z;
+                t[r] = make_uint4(0, 0, 0, 0);
+                if ((uint32_t)r < ny) {
+                    if (nz == 8u || q + 8 <= vol_end) {
+                        t[r] = ld16_a4(q);
+                    } else {
+                        const uint32_t *q32 = (const uint32_t *)q;
+                        t[r].x = q32[0];   // keep >= 1 (nz is even and nonzero)
+                        if (keep > 1) t[r].y = q32[1];
+                        if (keep > 2) t[r].z = q32[2];
+                    }
                 }
             }
         }
+        if (keep == 4u) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            D[4 * r] = t[r].x;
-            D[4 * r + 1] = keep > 1 ? t[r].y : 0u;
-            D[4 * r + 2] = keep > 2 ? t[r].z : 0u;
-            D[4 * r + 3] = keep > 3 ? t[r].w : 0u;
+            for (int r = 0; r < 8; r++) {
+                D[4 * r] = t[r].x;
+                D[4 * r + 1] = t[r].y;
+                D[4 * r + 2] = t[r].z;
+                D[4 * r + 3] = t[r].w;
+            }
+        } else {   // a block at a row end: clear the pairs past z
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                D[4 * r] = t[r].x;
+                D[4 * r + 1] = keep > 1 ? t[r].y : 0u;
+                D[4 * r + 2] = keep > 2 ? t[r].z : 0u;
+                D[4 * r + 3] = keep > 3 ? t[r].w : 0u;
+            }
         }
     } else {
         // Rows only 2-byte aligned (z odd, or a volume starting at 2 mod 4).  A
@@ -275,31 +294,49 @@ __device__ __forceinline__ void load_block_u32(const Item &it, uint32_t bx, uint
     // row loads by elements
     const uint32_t *vol_end = (const uint32_t *)it.vol_in + (uint64_t)it.x * it.y * it.z;
     uint4 t[16];
+    if (ny == 8u && p + 7ull * it.z + 8 <= vol_end) {   // every row's 32 bytes inside the volume
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-        const uint32_t *q = p + (uint64_t)r * it.z;
-        t[2 * r] = make_uint4(0, 0, 0, 0);
-        t[2 * r + 1] = make_uint4(0, 0, 0, 0);
-        if ((uint32_t)r < ny) {
-            if (nz == 8u || q + 8 <= vol_end) {
-                t[2 * r] = ld16_a4(q);
-                t[2 * r + 1] = ld16_a4(q + 4);
-            } else {
-                uint32_t e[8];
+        for (int r = 0; r < 8; r++) {
+            t[2 * r] = ld16_a4(p + (uint64_t)r * it.z);
+            t[2 * r + 1] = ld16_a4(p + (uint64_t)r * it.z + 4);
+        }
+    } else {
 #pragma unroll
-                for (int k = 0; k < 8; k++) e[k] = (uint32_t)k < nz ? q[k] : 0u;
-                t[2 * r] = make_uint4(e[0], e[1], e[2], e[3]);
-                t[2 * r + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+        for (int r = 0; r < 8; r++) {
+            const uint32_t *q = p + (uint64_t)r * it.z;
+            t[2 * r] = make_uint4(0, 0, 0, 0);
+            t[2 * r + 1] = make_uint4(0, 0, 0, 0);
+            if ((uint32_t)r < ny) {
+                if (nz == 8u || q + 8 <= vol_end) {
+                    t[2 * r] = ld16_a4(q);
+                    t[2 * r + 1] = ld16_a4(q + 4);
+                } else {
+                    uint32_t e[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) e[k] = (uint32_t)k < nz ? q[k] : 0u;
+                    t[2 * r] = make_uint4(e[0], e[1], e[2], e[3]);
+                    t[2 * r + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+                }
             }
         }
     }
+    if (nz == 8u) {
 #pragma unroll
-    for (int h = 0; h < 16; h++) {
-        const uint32_t k0 = 4u * (h & 1);   // column of the first element
-        V[4 * h] = k0 < nz ? t[h].x : 0u;
-        V[4 * h + 1] = k0 + 1 < nz ? t[h].y : 0u;
-        V[4 * h + 2] = k0 + 2 < nz ? t[h].z : 0u;
-        V[4 * h + 3] = k0 + 3 < nz ? t[h].w : 0u;
+        for (int h = 0; h < 16; h++) {
+            V[4 * h] = t[h].x;
+            V[4 * h + 1] = t[h].y;
+            V[4 * h + 2] = t[h].z;
+            V[4 * h + 3] = t[h].w;
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 16; h++) {
+            const uint32_t k0 = 4u * (h & 1);   // column of the first element
+            V[4 * h] = k0 < nz ? t[h].x : 0u;
+            V[4 * h + 1] = k0 + 1 < nz ? t[h].y : 0u;
+            V[4 * h + 2] = k0 + 2 < nz ? t[h].z : 0u;
+            V[4 * h + 3] = k0 + 3 < nz ? t[h].w : 0u;
+        }
     }
 }
 

@@ -30,6 +30,13 @@ def test_metrics_vs_numpy(gpu, n, offset):
     nz = a64 != 0
     if nz.any():
         assert s["max_rel_err"] == pytest.approx((err[nz] / a64[nz]).max(), rel=1e-6)
+    # normErr = sqrt(sum err^2), normErr_norm = normErr / sqrt(sum dec^2)
+    # (cuLSZ_utility.cu:900, 935-936: sum22 accumulates the decoded values)
+    norm = np.sqrt((err ** 2).sum())
+    assert s["norm_err"] == pytest.approx(norm, rel=1e-12)
+    dsq = (d64 ** 2).sum()
+    if dsq > 0:
+        assert s["norm_err_norm"] == pytest.approx(norm / np.sqrt(dsq), rel=1e-12)
 
 
 def test_metrics_of_a_lossy_round_trip(gpu):
