@@ -79,19 +79,21 @@ def time_codec(torch, lscomp, d_in, dims, bins, pool, d_out, d_dec, d_size, args
     for _ in range(args.warmup):
         comp()
     torch.cuda.synchronize()
-    c_wall, c_ev, _ = timed_loop(torch, comp, args.steps)
+    c_wall, c_ev, c_per = timed_loop(torch, comp, args.steps)
     s_bytes = int(d_size.item())
     dec = lambda: lscomp.decompress_async(d_out, s_bytes, dims, bins, d_dec)
     for _ in range(args.warmup):
         dec()
     torch.cuda.synchronize()
-    d_wall, d_ev, _ = timed_loop(torch, dec, args.steps)
+    d_wall, d_ev, d_per = timed_loop(torch, dec, args.steps)
     ok = lscomp.stream_status() == lscomp.OK
     res = {
         "compress_GBps": in_bytes / (c_wall * 1e-3) / 1e9,
         "decompress_GBps": in_bytes / (d_wall * 1e-3) / 1e9,
         "compress_ms": c_wall, "decompress_ms": d_wall,
         "compress_kernel_ms": c_ev, "decompress_kernel_ms": d_ev,
+        "compress_kernel_ms_min_max": [min(c_per), max(c_per)],
+        "decompress_kernel_ms_min_max": [min(d_per), max(d_per)],
         "stream_bytes": s_bytes, "ratio": in_bytes / s_bytes,
         "compress_hbm_frac": (in_bytes + s_bytes) / (c_ev * 1e-3) / HBM_PEAK,
         "decompress_hbm_frac": (in_bytes + s_bytes) / (d_ev * 1e-3) / HBM_PEAK,
