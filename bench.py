@@ -42,18 +42,44 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+_SPIN_PER_MS = []
+
+
+def gpu_busy(torch, ms=1.0):
+    """Keep the stream busy for about `ms` (torch's spin kernel) so that the launches
+    the host enqueues next queue up behind it: an event pair around a launch then
+    measures the kernel, not a pause of the host before the launch was issued.  The
+    spin's unit is calibrated once (its counter's rate depends on the platform)."""
+    try:
+        if not _SPIN_PER_MS:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            torch.cuda._sleep(100000)
+            torch.cuda.synchronize()
+            _SPIN_PER_MS.append(100000 / max((time.perf_counter() - t0) * 1e3, 1e-3))
+        torch.cuda._sleep(max(1, int(ms * _SPIN_PER_MS[0])))
+    except (AttributeError, RuntimeError):
+        pass
+
+
 def timed_loop(torch, fn, steps):
-    """Average ms of fn() over `steps` launches on the current stream, plus per-launch event times."""
+    """Wall-clock ms per launch of fn() over `steps` launches on the current stream
+    (host enqueue included), then the mean and per-launch kernel times from event
+    pairs around `steps` more launches queued behind a spin kernel."""
     stream = torch.cuda.current_stream()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    gpu_busy(torch)
     for a, b in evs:
         a.record(stream)
         fn()
         b.record(stream)
     torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / steps
     per = [a.elapsed_time(b) for a, b in evs]
     return wall, sum(per) / len(per), per
 
