@@ -316,6 +316,19 @@ typedef struct lscomp_error_stats {
 int lscomp_error_stats_u16(const uint16_t *d_orig, const uint16_t *d_dec, size_t n, lscomp_error_stats *out,
                            hipStream_t stream);
 
+/* Mean SSIM of two uint16 volumes (x, y, z), z fastest, into *ssim: replaces
+ * the reference's computeSSIM(ori, dec, x, y, z) (lsCOMP/src/cuLSZ_utility.cu:
+ * 844-886, windows 757-842): 7x7x7 windows moved by 2 along every axis,
+ * luminance * contrast * structure with c1 = (0.01 R)^2, c2 = (0.03 R)^2 from
+ * the original's range R in the window (0.01^2, 0.03^2 when R = 0), averaged
+ * over the windows.  Window moments are exact (the reference rounds two-pass
+ * double sums), so results agree to about 1e-12 relative.  Where the
+ * reference prints an error and returns 0/0 (a dimension below 7), this
+ * returns LSCOMP_ERR_INVALID_ARG.  Synchronous on `stream`; allocates one
+ * double per workgroup. */
+int lscomp_ssim_u16(const uint16_t *d_orig, const uint16_t *d_dec, lscomp_dim3 dims, double *ssim,
+                    hipStream_t stream);
+
 /* ---- uint32 volumes --------------------------------------------------------
  * Replace cuLSZ_compression_uint32_bsize64 / cuLSZ_decompression_uint32_bsize64
  * (lsCOMP/include/cuLSZ_entry.h:8-9, lsCOMP/src/cuLSZ_entry.cu:18-91): same

@@ -78,6 +78,8 @@ class Oracle:
         lib.oracle_decompress_u32.restype = ctypes.c_int
         lib.oracle_decompress_u32.argtypes = [_u8p, _u64, _u32, _u32, _u32, _u32p, _u32p]
         lib.oracle_max_threads.restype = ctypes.c_int
+        lib.oracle_ssim_u16.restype = ctypes.c_double
+        lib.oracle_ssim_u16.argtypes = [_u16p, _u16p, _u32, _u32, _u32]
         self.lib = lib
 
     # -- geometry ---------------------------------------------------------
@@ -117,6 +119,15 @@ class Oracle:
         return v
 
     # -- whole volume -----------------------------------------------------
+    def ssim(self, ori: np.ndarray, dec: np.ndarray) -> float:
+        """Mean SSIM of two uint16 volumes (x, y, z), as the reference's computeSSIM
+        (lsCOMP/src/cuLSZ_utility.cu:757-886) on their float values; NaN when a dimension is below 7."""
+        a = np.ascontiguousarray(ori, dtype=np.uint16)
+        b = np.ascontiguousarray(dec, dtype=np.uint16)
+        assert a.shape == b.shape and a.ndim == 3
+        x, y, z = a.shape
+        return float(self.lib.oracle_ssim_u16(a.ctypes.data_as(_u16p), b.ctypes.data_as(_u16p), x, y, z))
+
     def compress(self, vol: np.ndarray, bins, p: float, omp: bool = False, capacity: int | None = None):
         """Returns (status, stream bytes as np.uint8 array, required size)."""
         vol = np.ascontiguousarray(vol, dtype=np.uint16)

@@ -17,6 +17,7 @@
  *   payload placement (scan)         lsCOMP/src/cuLSZ_kernel.cu:962-1097
  *   quantize + bit-plane packing     lsCOMP/src/cuLSZ_kernel.cu:1099-1277
  *   decode: headers, unpack, store   lsCOMP/src/cuLSZ_kernel.cu:1322-1640
+ *   SSIM of two volumes              lsCOMP/src/cuLSZ_utility.cu:757-886
  *
  * Stream format (block order b = 0..B-1, B = x*ceil(y/8)*ceil(z/8)):
  *   [header byte of every block][zero pad to R = 4*ceil(B/4)][payload(0)]...
@@ -49,6 +50,7 @@
  * ORACLE_MUTANT=n (compile-time) plants one plausible mistake; the test suite
  * builds each mutant and checks that its pins reject it.
  */
+#include <math.h>
 #include <stdint.h>
 #include <stddef.h>
 #include <string.h>
@@ -566,4 +568,88 @@ int oracle_max_threads(void)
 #else
     return 1;
 #endif
+}
+
+/* ---- SSIM (cuLSZ_utility.cu:757-886) -------------------------------------
+ * Mean structural similarity of two volumes over 7x7x7 windows moved by 2
+ * along every axis (size0 fastest), as the reference's computeSSIM: the
+ * uint16 values are the reference's float samples; means and the two-pass
+ * variances and covariance in double, c1 and c2 from the original's range in
+ * the window (0.01^2 and 0.03^2 when it is 0), c3 = c2 / 2.  No window fits
+ * when a dimension is below 7: 0/0, as the reference. */
+static double ssim_window(const uint16_t *data, const uint16_t *other, size_t size1, size_t size0,
+                          size_t o0, size_t o1, size_t o2)
+{
+    const int w = 7;
+    int np = 0;
+    const size_t first = o0 + size0 * (o1 + size1 * o2);
+    float xMin = (float)data[first], xMax = xMin, yMin = (float)other[first], yMax = yMin;
+    double xSum = 0, ySum = 0;
+    for (size_t i2 = o2; i2 < o2 + w; i2++)
+        for (size_t i1 = o1; i1 < o1 + w; i1++)
+            for (size_t i0 = o0; i0 < o0 + w; i0++) {
+                const size_t k = i0 + size0 * (i1 + size1 * i2);
+                const float a = (float)data[k], b = (float)other[k];
+                np++;
+                if (xMin > a) xMin = a;
+                if (xMax < a) xMax = a;
+                if (yMin > b) yMin = b;
+                if (yMax < b) yMax = b;
+                xSum += a;
+                ySum += b;
+            }
+    const double xMean = xSum / np, yMean = ySum / np;
+    double var_x = 0, var_y = 0, var_xy = 0;
+    for (size_t i2 = o2; i2 < o2 + w; i2++)
+        for (size_t i1 = o1; i1 < o1 + w; i1++)
+            for (size_t i0 = o0; i0 < o0 + w; i0++) {
+                const size_t k = i0 + size0 * (i1 + size1 * i2);
+                const float a = (float)data[k], b = (float)other[k];
+                var_x += (a - xMean) * (a - xMean);
+                var_y += (b - yMean) * (b - yMean);
+                var_xy += (a - xMean) * (b - yMean);
+            }
+    var_x /= np;
+    var_y /= np;
+    var_xy /= np;
+    const double xSigma = sqrt(var_x), ySigma = sqrt(var_y), xyCov = var_xy;
+#if ORACLE_MUTANT == 10
+    const float range = yMax - yMin;   /* planted: the decoded volume's range */
+#else
+    const float range = xMax - xMin;
+#endif
+    (void)yMin;
+    (void)yMax;
+    double c1, c2;
+    if (range == 0) {
+        c1 = 0.01 * 0.01;
+        c2 = 0.03 * 0.03;
+    } else {
+        c1 = 0.01 * 0.01 * range * range;
+        c2 = 0.03 * 0.03 * range * range;
+    }
+    const double c3 = c2 / 2;
+    const double luminance = (2 * xMean * yMean + c1) / (xMean * xMean + yMean * yMean + c1);
+    const double contrast = (2 * xSigma * ySigma + c2) / (xSigma * xSigma + ySigma * ySigma + c2);
+    const double structure = (xyCov + c3) / (xSigma * ySigma + c3);
+    return luminance * contrast * structure;
+}
+
+double oracle_ssim_u16(const uint16_t *ori, const uint16_t *dec, uint32_t size2, uint32_t size1, uint32_t size0)
+{
+#if ORACLE_MUTANT == 9
+    const size_t shift = 1;   /* planted: windows moved by 1 */
+#else
+    const size_t shift = 2;
+#endif
+    const size_t w = 7;
+    double sum = 0;
+    uint64_t nw = 0;
+    for (size_t o2 = 0; o2 + w <= size2; o2 += shift)
+        for (size_t o1 = 0; o1 + w <= size1; o1 += shift)
+            for (size_t o0 = 0; o0 + w <= size0; o0 += shift) {
+                nw++;
+                sum += ssim_window(ori, dec, size1, size0, o0, o1, o2);
+            }
+    return sum / (double)nw;
 }

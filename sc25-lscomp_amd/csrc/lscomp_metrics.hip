@@ -6,12 +6,14 @@
 // sqrt(sum dec^2), nrmse = sqrt(mse)/range; plus the maximum absolute error
 // and the maximum pointwise relative error over nonzero originals.  Sums are
 // exact 64-bit integers (uint16 data), reduced per wave and then atomically.
-// One pass, HBM-bound: 4 bytes read per element pair.  Also the plain copy
-// used to measure copy bandwidth and to calibrate the HBM byte counters.
+// One pass, HBM-bound: 4 bytes read per element pair.  Also the mean SSIM
+// of the reference's computeSSIM (cuLSZ_utility.cu:757-886), and the plain
+// copy used to measure copy bandwidth and to calibrate the HBM byte counters.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "../../include/lscomp.h"
 #include "lscomp_host.h"
@@ -115,6 +117,169 @@ __global__ void copy_tail_kernel(uint8_t *dst, const uint8_t *src, uint32_t n)
     if (threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
 }
 
+// ---------------------------------------------------------------------------
+// SSIM (cuLSZ_utility.cu:757-886): the mean over 7x7x7 windows moved by 2
+// along every axis of luminance * contrast * structure, with c1 = (0.01 R)^2,
+// c2 = (0.03 R)^2 (R: the original's range in the window; 0.01^2 and 0.03^2
+// when R = 0) and c3 = c2 / 2.
+//
+// Separable window sums instead of the reference's two passes over every
+// window's 343 points.  A workgroup owns 64 x 4 windows along (size0, size1)
+// and up to 64 windows along size2, and walks that slab plane by plane: each
+// plane's 13 x 133 patch of both volumes goes to LDS; row sums over 7 columns
+// (at column stride 2) for 13 rows x 64 windows; each thread sums 7 rows of
+// its window into the plane's moments P; along size2 a window is the sum of
+// three plane pairs and one plane (planes 2k .. 2k+6), kept in registers.
+// Moments are exact: integer sums of values, double sums of squares and
+// products (integers below 2^53), so the variances and the covariance are
+// n*S_xx - S_x^2 (exact 64-bit integers) over n^2, correctly rounded, where the
+// reference rounds a two-pass double sum.  Each workgroup writes one partial
+// sum, reduced in a fixed order; the host adds the partials in order, so the
+// result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSsimT0 = 64, kSsimT1 = 4, kSsimChunk = 64;   // windows per workgroup along size0, size1, size2
+constexpr uint32_t kSsimPC = 2 * kSsimT0 + 5, kSsimPR = 2 * kSsimT1 + 5;   // patch columns (133) and rows (13)
+constexpr uint32_t kSsimPitch = 136;                                       // u16 per patch row (even, >= 134)
+
+struct SsimMom {
+    uint32_t sx, sy, mn, mx;   // sums of both volumes, min and max of the original
+    double sxx, syy, sxy;      // exact integer sums
+};
+
+__device__ __forceinline__ SsimMom mom_add(const SsimMom &a, const SsimMom &b)
+{
+    return {a.sx + b.sx, a.sy + b.sy, a.mn < b.mn ? a.mn : b.mn, a.mx > b.mx ? a.mx : b.mx,
+            a.sxx + b.sxx, a.syy + b.syy, a.sxy + b.sxy};
+}
+
+__device__ double ssim_of(const SsimMom &m)
+{
+#pragma clang fp contract(off)
+    const double np = 343.0;
+    const double x_mean = (double)m.sx / np, y_mean = (double)m.sy / np;
+    const int64_t sx = m.sx, sy = m.sy;
+    const double n2 = np * np;
+    const double var_x = (double)(343 * (int64_t)m.sxx - sx * sx) / n2;
+    const double var_y = (double)(343 * (int64_t)m.syy - sy * sy) / n2;
+    const double cov = (double)(343 * (int64_t)m.sxy - sx * sy) / n2;
+    const double x_sigma = sqrt(var_x), y_sigma = sqrt(var_y);
+    const float range = (float)(m.mx - m.mn);
+    double c1, c2;
+    if (range == 0.0f) {
+        c1 = 0.01 * 0.01;
+        c2 = 0.03 * 0.03;
+    } else {
+        c1 = 0.01 * 0.01 * range * range;
+        c2 = 0.03 * 0.03 * range * range;
+    }
+    const double c3 = c2 / 2;
+    const double luminance = (2 * x_mean * y_mean + c1) / (x_mean * x_mean + y_mean * y_mean + c1);
+    const double contrast = (2 * x_sigma * y_sigma + c2) / (x_sigma * x_sigma + y_sigma * y_sigma + c2);
+    const double structure = (cov + c3) / (x_sigma * y_sigma + c3);
+    return luminance * contrast * structure;
+}
+
+// One workgroup per (size0 tile, size1 tile, size2 chunk) of windows; grid
+// nb0 * nb1 * nchunks workgroups of 256 threads; partial[blockIdx.x] = the
+// sum of the SSIM of its windows.
+__global__ __launch_bounds__(256) void ssim_kernel(const uint16_t *__restrict__ ori, const uint16_t *__restrict__ dec,
+                                                   uint32_t X, uint32_t Y, uint32_t Z, uint32_t n0, uint32_t n1,
+                                                   uint32_t n2, uint32_t nb0, uint32_t nb1, double *partial)
+{
+    __shared__ __align__(16) uint16_t xs[kSsimPR][kSsimPitch], ys[kSsimPR][kSsimPitch];
+    __shared__ uint32_t r_sx[kSsimPR][kSsimT0], r_sy[kSsimPR][kSsimT0], r_mn[kSsimPR][kSsimT0],
+        r_mx[kSsimPR][kSsimT0];
+    __shared__ double r_sxx[kSsimPR][kSsimT0], r_syy[kSsimPR][kSsimT0], r_sxy[kSsimPR][kSsimT0];
+    __shared__ double wsum[4];
+    const uint32_t tid = threadIdx.x, t0 = tid & 63u, t1 = tid >> 6;
+    uint32_t blk = blockIdx.x;
+    const uint32_t b0 = blk % nb0;
+    blk /= nb0;
+    const uint32_t b1 = blk % nb1, c2 = blk / nb1;
+    const uint32_t o0b = b0 * kSsimT0, o1b = b1 * kSsimT1, o2b = c2 * kSsimChunk;
+    const uint32_t o2e = o2b + kSsimChunk < n2 ? o2b + kSsimChunk : n2;
+    const uint64_t col0 = 2ull * o0b, row0 = 2ull * o1b;
+    const bool mine = o0b + t0 < n0 && o1b + t1 < n1;
+    // planes [2 o2b, 2 (o2e - 1) + 7): the last is at most X - 1 since o2e <= n2
+    const uint32_t pb = 2u * o2b, pe = 2u * (o2e - 1u) + 7u;
+    (void)X;
+
+    SsimMom q0{}, q1{}, q2{}, pev{};   // the last three plane pairs, the last even plane
+    double acc = 0.0;
+    for (uint32_t p = pb; p < pe; p++) {
+        // 1. the plane's patch (zeros outside the volume: only windows that
+        //    are not counted read them)
+        for (uint32_t e = tid; e < kSsimPR * kSsimPC; e += 256u) {
+            const uint32_t r = e / kSsimPC, c = e - r * kSsimPC;
+            const uint64_t gy = row0 + r, gz = col0 + c;
+            uint16_t a = 0, b = 0;
+            if (gy < Y && gz < Z) {
+                const uint64_t i = ((uint64_t)p * Y + gy) * Z + gz;
+                a = ori[i];
+                b = dec[i];
+            }
+            xs[r][c] = a;
+            ys[r][c] = b;
+        }
+        __syncthreads();
+        // 2. row sums: 7 columns from column 2w, read as four aligned pairs
+        for (uint32_t e = tid; e < kSsimPR * kSsimT0; e += 256u) {
+            const uint32_t r = e >> 6, w = e & 63u;
+            const uint32_t *xr = (const uint32_t *)&xs[r][0] + w, *yr = (const uint32_t *)&ys[r][0] + w;
+            uint32_t sx = 0, sy = 0, mn = 0xffffu, mx = 0;
+            double sxx = 0, syy = 0, sxy = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t xw = xr[k], yw = yr[k];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (k == 3 && h == 1) break;   // 7 columns
+                    const uint32_t a = h ? xw >> 16 : xw & 0xffffu, b = h ? yw >> 16 : yw & 0xffffu;
+                    sx += a;
+                    sy += b;
+                    mn = a < mn ? a : mn;
+                    mx = a > mx ? a : mx;
+                    const double da = (double)a, db = (double)b;
+                    sxx = fma(da, da, sxx);
+                    syy = fma(db, db, syy);
+                    sxy = fma(da, db, sxy);
+                }
+            }
+            r_sx[r][w] = sx;
+            r_sy[r][w] = sy;
+            r_mn[r][w] = mn;
+            r_mx[r][w] = mx;
+            r_sxx[r][w] = sxx;
+            r_syy[r][w] = syy;
+            r_sxy[r][w] = sxy;
+        }
+        __syncthreads();
+        // 3. this thread's window in the plane: rows 2 t1 .. 2 t1 + 6
+        SsimMom P{0, 0, 0xffffu, 0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (uint32_t k = 0; k < 7; k++) {
+            const uint32_t r = 2u * t1 + k;
+            P = mom_add(P, SsimMom{r_sx[r][t0], r_sy[r][t0], r_mn[r][t0], r_mx[r][t0], r_sxx[r][t0], r_syy[r][t0],
+                                   r_sxy[r][t0]});
+        }
+        // 4. along size2: window k covers pairs k, k+1, k+2 and plane 2k + 6
+        const uint32_t j = p - pb;
+        if ((j & 1u) == 0) {
+            if (j >= 6 && mine) acc += ssim_of(mom_add(mom_add(q0, q1), mom_add(q2, P)));
+            pev = P;
+        } else {
+            q0 = q1;
+            q1 = q2;
+            q2 = mom_add(pev, P);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((tid & 63u) == 0) wsum[t1] = acc;
+    __syncthreads();
+    if (tid == 0) partial[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
 }  // namespace lsc
 
 extern "C" int lscomp_copy_bytes(void *d_dst, const void *d_src, size_t bytes, hipStream_t stream)
@@ -173,5 +338,35 @@ extern "C" int lscomp_error_stats_u16(const uint16_t *d_orig, const uint16_t *d_
     out->norm_err = std::sqrt((double)host.sum_sq_err);
     out->norm_err_norm = out->norm_err / std::sqrt((double)host.sum_sq_dec);
     out->nrmse = std::sqrt(out->mse) / out->range;
+    return LSCOMP_OK;
+}
+
+extern "C" int lscomp_ssim_u16(const uint16_t *d_orig, const uint16_t *d_dec, lscomp_dim3 dims, double *ssim,
+                               hipStream_t stream)
+{
+    using namespace lsc;
+    if (!d_orig || !d_dec || !ssim || (((uintptr_t)d_orig | (uintptr_t)d_dec) & 1)) return LSCOMP_ERR_INVALID_ARG;
+    if (dims.x < 7 || dims.y < 7 || dims.z < 7) return LSCOMP_ERR_INVALID_ARG;   // no window fits
+    const uint32_t n0 = (dims.z - 7) / 2 + 1, n1 = (dims.y - 7) / 2 + 1, n2 = (dims.x - 7) / 2 + 1;
+    const uint32_t nb0 = (n0 + kSsimT0 - 1) / kSsimT0, nb1 = (n1 + kSsimT1 - 1) / kSsimT1;
+    const uint32_t nc = (n2 + kSsimChunk - 1) / kSsimChunk;
+    const uint64_t blocks = (uint64_t)nb0 * nb1 * nc;
+    if (blocks > 0x7fffffffull) return LSCOMP_ERR_TOO_LARGE;
+    double *partial = nullptr;
+    int st = hip_fail(hipMalloc((void **)&partial, blocks * sizeof(double)), "hipMalloc");
+    if (st) return st;
+    std::vector<double> host(blocks);
+    hipLaunchKernelGGL(ssim_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, d_orig, d_dec, dims.x, dims.y,
+                       dims.z, n0, n1, n2, nb0, nb1, partial);
+    st = hip_fail(hipGetLastError(), "ssim launch");
+    if (!st)
+        st = hip_fail(hipMemcpyAsync(host.data(), partial, blocks * sizeof(double), hipMemcpyDeviceToHost, stream),
+                      "hipMemcpyAsync");
+    if (!st) st = hip_fail(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    (void)hipFree(partial);
+    if (st) return st;
+    double sum = 0.0;
+    for (double v : host) sum += v;
+    *ssim = sum / ((double)n0 * (double)n1 * (double)n2);
     return LSCOMP_OK;
 }

@@ -74,7 +74,7 @@ EXPORTS = (
     "lscomp_plan_launch_graph", "lscomp_plan_device_sizes", "lscomp_plan_sizes", "lscomp_plan_status",
     "lscomp_plan_destroy", "lscomp_server_create", "lscomp_server_submit_u16", "lscomp_server_poll",
     "lscomp_server_release", "lscomp_server_drain", "lscomp_server_replay_u16", "lscomp_server_destroy",
-    "lscomp_autotune_u16", "lscomp_set_decode_prepass", "lscomp_error_stats_u16", "lscomp_copy_bytes",
+    "lscomp_autotune_u16", "lscomp_set_decode_prepass", "lscomp_error_stats_u16", "lscomp_ssim_u16", "lscomp_copy_bytes",
     "lscomp_max_compressed_bytes_u32", "lscomp_compress_u32", "lscomp_compress_u32_async",
     "lscomp_decompress_u32", "lscomp_decompress_u32_async",
     "lscomp_container_bytes", "lscomp_container_pack", "lscomp_container_parse",
@@ -231,6 +231,8 @@ def _load():
     lib.lscomp_server_destroy.argtypes = [vp]
     lib.lscomp_error_stats_u16.restype = ctypes.c_int
     lib.lscomp_error_stats_u16.argtypes = [vp, vp, sz, ctypes.POINTER(ErrorStats), vp]
+    lib.lscomp_ssim_u16.restype = ctypes.c_int
+    lib.lscomp_ssim_u16.argtypes = [vp, vp, Dim3, ctypes.POINTER(ctypes.c_double), vp]
     lib.lscomp_copy_bytes.restype = ctypes.c_int
     lib.lscomp_copy_bytes.argtypes = [vp, vp, sz, vp]
     lib.lscomp_set_decode_prepass.restype = ctypes.c_int
@@ -565,6 +567,22 @@ def error_stats(orig, dec, stream=None) -> dict:
     if st != OK:
         raise LscompError(st, "lscomp_error_stats_u16")
     return {name: getattr(out, name) for name, _ in ErrorStats._fields_ if name != "reserved"}
+
+
+def ssim(orig, dec, dims, stream=None) -> float:
+    """Mean SSIM of two device volumes of 2-byte elements (uint16 bits) of shape
+    dims = (x, y, z), z fastest: the reference's computeSSIM
+    (lsCOMP/src/cuLSZ_utility.cu:757-886) on the GPU."""
+    x, y, z = map(int, dims)
+    if orig.numel() != x * y * z or dec.numel() != x * y * z or orig.element_size() != 2 \
+            or dec.element_size() != 2:
+        raise ValueError("orig and dec must be 2-byte tensors of dims' size")
+    out = ctypes.c_double(0.0)
+    st = _lib.lscomp_ssim_u16(ctypes.c_void_p(orig.data_ptr()), ctypes.c_void_p(dec.data_ptr()), Dim3(x, y, z),
+                              ctypes.byref(out), _stream_handle(stream))
+    if st != OK:
+        raise LscompError(st, "lscomp_ssim_u16")
+    return out.value
 
 
 def copy_bytes(dst, src, stream=None) -> None:

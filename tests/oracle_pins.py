@@ -353,12 +353,64 @@ def pin_u32_matches_u16(orc):
                                   orc.decompress_u32(b[1], vol.shape, bins)[1])
 
 
+# ---- SSIM (cuLSZ_utility.cu:757-886) ----------------------------------------
+
+def _ssim_numpy(a, b):
+    """Mean SSIM over 7x7x7 windows at stride 2, vectorized with float64
+    moments (an independent formulation of the reference's per-window loops)."""
+    from numpy.lib.stride_tricks import sliding_window_view as swv
+    x = swv(a.astype(np.float64), (7, 7, 7))[::2, ::2, ::2].reshape(-1, 343)
+    y = swv(b.astype(np.float64), (7, 7, 7))[::2, ::2, ::2].reshape(-1, 343)
+    mx, my = x.mean(1), y.mean(1)
+    vx, vy = ((x - mx[:, None]) ** 2).mean(1), ((y - my[:, None]) ** 2).mean(1)
+    cxy = ((x - mx[:, None]) * (y - my[:, None])).mean(1)
+    r = x.max(1) - x.min(1)
+    c1 = np.where(r == 0, 1e-4, 1e-4 * r * r)
+    c2 = np.where(r == 0, 9e-4, 9e-4 * r * r)
+    sx, sy = np.sqrt(vx), np.sqrt(vy)
+    ssim = ((2 * mx * my + c1) / (mx * mx + my * my + c1) * (2 * sx * sy + c2) / (vx + vy + c2)
+            * (cxy + c2 / 2) / (sx * sy + c2 / 2))
+    return float(ssim.mean())
+
+
+def pin_ssim(orc):
+    from fractions import Fraction as F
+    rng = np.random.default_rng(77)
+    # one window, closed form: x = 10 everywhere, y = 10 but one sample 17.
+    # Range 0 -> c1 = 1e-4, c2 = 9e-4; sigma_x = cov = 0 -> structure 1;
+    # mean_y = 10 + 1/49, var_y = 49 * 342 / 343^2.
+    x = np.full((7, 7, 7), 10, np.uint16)
+    y = x.copy(); y[3, 2, 5] = 17
+    my, vy = 10 + F(1, 49), F(49 * 342, 343 * 343)
+    c1, c2 = F(1, 10 ** 4), F(9, 10 ** 4)
+    want = float((20 * my + c1) / (100 + my * my + c1) * c2 / (vy + c2))
+    assert abs(orc.ssim(x, y) - want) <= 1e-13 * want
+    # identical volumes: 1
+    v = rng.integers(0, 65536, (9, 12, 17)).astype(np.uint16)
+    assert abs(orc.ssim(v, v) - 1.0) < 1e-13
+    # no window fits: 0/0
+    assert np.isnan(orc.ssim(v[:, :6], v[:, :6]))
+    # against the vectorized formulation: uniform noise, sparse spikes (windows
+    # whose original is constant), shapes with odd and even remainders
+    cases = []
+    a = rng.integers(0, 65536, (9, 11, 16)).astype(np.uint16)
+    cases.append((a, np.clip(a.astype(np.int64) + rng.integers(-300, 301, a.shape), 0, 65535).astype(np.uint16)))
+    s = np.zeros((8, 13, 20), np.uint16); s[rng.random(s.shape) < 0.02] = 900
+    cases.append((s, (s // 3 * 3).astype(np.uint16)))
+    c = rng.poisson(2.0, (10, 10, 10)).astype(np.uint16)
+    cases.append((c, rng.poisson(2.0, c.shape).astype(np.uint16)))
+    for a, b in cases:
+        got, want = orc.ssim(a, b), _ssim_numpy(a, b)
+        assert abs(got - want) <= 1e-12 * abs(want), (a.shape, got, want)
+
+
 ALL_PINS = [pin_kat_bytes, pin_kat_decode, pin_onehot_plain, pin_onehot_pooled, pin_pool_threshold,
             pin_bin_choice, pin_rate_edges, pin_lossless_roundtrip, pin_stream_structure, pin_lossy_bounds,
             pin_header_locality, pin_frame_prefix, pin_omp_equals_serial, pin_u32_kat, pin_u32_structure,
-            pin_u32_matches_u16]
+            pin_u32_matches_u16, pin_ssim]
 
 # planted mistakes in oracle/lscomp_oracle.c (ORACLE_MUTANT=n)
 MUTANTS = {1: "plain bits LSB-first", 2: "plain block transposed", 3: "pool on >=", 4: "bin counts use <=",
            5: "no 4-byte header pad", 6: "pair average rounds up", 7: "rate one too high",
-           8: "pooled decode drops the second pair element"}
+           8: "pooled decode drops the second pair element", 9: "SSIM windows moved by 1",
+           10: "SSIM constants from the decoded range"}

@@ -67,3 +67,69 @@ def test_copy_bytes(gpu, nbytes):
         gpu.copy_bytes(dst[:nbytes - 1], src)
     with pytest.raises(gpu.LscompError):
         gpu.copy_bytes(dst[1:nbytes + 1], src)   # misaligned destination
+
+
+# ---- SSIM (cuLSZ_utility.cu:757-886) against the oracle's restatement --------
+
+def _noisy(a, rng, amp):
+    return np.clip(a.astype(np.int64) + rng.integers(-amp, amp + 1, a.shape), 0, 65535).astype(np.uint16)
+
+
+def _dev(v):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(v).view(np.int16)).cuda()
+
+
+# one window; tiles of 64 x 4 windows along z, y and chunks of 64 along x,
+# with odd and even remainders; a shape whose windows end one short of z
+@pytest.mark.parametrize("shape", [(7, 7, 7), (9, 20, 140), (20, 33, 300), (140, 9, 9), (8, 14, 8), (11, 7, 270)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_ssim_vs_oracle(gpu, orc, shape):
+    rng = np.random.default_rng(sum(shape))
+    a = rng.integers(0, 65536, shape).astype(np.uint16)
+    b = _noisy(a, rng, 2000)
+    got, want = gpu.ssim(_dev(a), _dev(b), shape), orc.ssim(a, b)
+    assert got == pytest.approx(want, rel=1e-10, abs=1e-13)
+
+
+def test_ssim_of_lossy_round_trips(gpu, orc):
+    """CSSI- and XPCS-like frames through the codec: sparse data, windows whose
+    original is constant (range 0), and the decoded volume's quantization."""
+    for kind, shape, bins, p in (("cssi", (12, 200, 260), (3, 5, 10, 15), 0.5),
+                                 ("xpcs", (9, 64, 520), (3, 3, 3, 3), 0.5),
+                                 ("cssi", (10, 90, 77), (10, 10, 10, 10), 1.0)):
+        vol = synth(kind, shape, seed=9)
+        d = _dev(vol)
+        out, n = gpu.compress(d, shape, bins, p)
+        dec = gpu.decompress(out, n, shape, bins)
+        got = gpu.ssim(d, dec, shape)
+        want = orc.ssim(vol, dec.cpu().numpy().view(np.uint16))
+        assert got == pytest.approx(want, rel=1e-10, abs=1e-13), kind
+
+
+def test_ssim_medium_volume(gpu, orc):
+    """40 x 300 x 400: 492 k windows, many workgroups and chunks."""
+    shape = (40, 300, 400)
+    rng = np.random.default_rng(40)
+    a = synth("xpcs", shape, seed=40)
+    b = _noisy(a, rng, 3)
+    got, want = gpu.ssim(_dev(a), _dev(b), shape), orc.ssim(a, b)
+    assert got == pytest.approx(want, rel=1e-10, abs=1e-13)
+
+
+def test_ssim_identity_at_full_frame_size(gpu):
+    """SSIM(v, v) = 1 and a lossless round trip gives 1, on 64 CSSI frames of
+    1813 x 1558 (20 M windows)."""
+    shape = (64, 1813, 1558)
+    vol = synth("cssi", shape, seed=2)
+    d = _dev(vol)
+    assert abs(gpu.ssim(d, d, shape) - 1.0) < 1e-12
+    out, n = gpu.compress(d, shape, (1, 1, 1, 1), 1.0)
+    assert abs(gpu.ssim(d, gpu.decompress(out, n, shape), shape) - 1.0) < 1e-12
+
+
+def test_ssim_rejects_volumes_without_a_window(gpu):
+    import lscomp
+    v = _dev(np.zeros((6, 10, 10), np.uint16))
+    with pytest.raises(lscomp.LscompError):
+        gpu.ssim(v, v, (6, 10, 10))
