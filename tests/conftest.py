@@ -66,6 +66,8 @@ def block_host():
     u32p = ctypes.POINTER(ctypes.c_uint32)
     lib.bh_encode.restype = ctypes.c_uint32
     lib.bh_encode.argtypes = [u16p, u32p, ctypes.c_float, u8p, u32p]
+    lib.bh_set_planes_form.restype = None
+    lib.bh_set_planes_form.argtypes = [ctypes.c_int]
     lib.bh_decode.restype = None
     lib.bh_decode.argtypes = [ctypes.c_uint32, u8p, u32p, u16p]
     lib.bh_encode_u32.restype = ctypes.c_uint32

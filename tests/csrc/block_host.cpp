@@ -16,6 +16,12 @@ static void pack_rows(const uint16_t v[64], uint32_t D[32])
 
 extern "C" {
 
+// Form of the bit-plane code: 0 by each block's plane count (as a wave of
+// equal blocks on the GPU), 1 one plane at a time, 2 all 8 planes of a group
+// at once (what a block gets on the GPU when another lane of its wave needs
+// 3 planes or more).
+void bh_set_planes_form(int form) { g_planes_form = form; }
+
 // Header byte and payload bytes of one block; payload written to out (<= 128 B).
 uint32_t bh_encode(const uint16_t v[64], const uint32_t bins[4], float p, uint8_t *out, uint32_t *hdr)
 {

@@ -39,11 +39,14 @@ def _blocks(rng, n):
     return out
 
 
+# planes form: by each block's plane count, one plane at a time, all 8 at once
+@pytest.mark.parametrize("form", [0, 1, 2], ids=["by_count", "per_plane", "transpose"])
 @pytest.mark.parametrize("mode", sorted(MODES))
-def test_encode_decode_vs_oracle(block_host, orc, mode):
+def test_encode_decode_vs_oracle(block_host, orc, mode, form):
     bins, p = MODES[mode]
     b = np.array(bins, np.uint32)
     rng = np.random.default_rng(sum(map(ord, mode)))
+    block_host.bh_set_planes_form(form)
     for v in _blocks(rng, 3000):
         h, _ = orc.block_header(v.astype(np.uint32), b, p)
         ref = orc.encode_block(v.astype(np.uint32), h, b)
@@ -60,9 +63,11 @@ def test_encode_decode_vs_oracle(block_host, orc, mode):
         assert np.array_equal(dec, orc.decode_block(h, ref, b).astype(np.uint16))
 
 
-def test_decode_random_payloads(block_host, orc):
+@pytest.mark.parametrize("form", [0, 1, 2], ids=["by_count", "per_plane", "transpose"])
+def test_decode_random_payloads(block_host, orc, form):
     """Arbitrary (also non-canonical) headers and payloads decode like the oracle, incl. mod-2^16 products."""
     rng = np.random.default_rng(99)
+    block_host.bh_set_planes_form(form)
     for bins in [(1, 1, 1, 1), (3, 5, 10, 15), (7, 300, 5000, 70000)]:
         b = np.array(bins, np.uint32)
         for _ in range(2000):
