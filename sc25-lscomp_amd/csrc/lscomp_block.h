@@ -409,8 +409,9 @@ LSC_HD void columns_of_planes8(uint32_t P[8])
 // (its costly waves are the few with rates of 10-14 around the bright spots);
 // decompression gains 0.2 % from 5 planes on.  Lossless compression lost 5 %
 // with the transpose at any threshold from 3 to 6, so it keeps the plane
-// loop.  The host build (tests) takes the block's own count, or forces
-// either form.
+// loop, and so does uint32 compression (its 206-register kernel lost 9-12 %
+// with the transpose).  The host build (tests) takes the block's own count,
+// or forces either form.
 constexpr uint32_t kPlanesEncode = 3, kPlanesDecode = 5;
 #if !defined(__HIP_DEVICE_COMPILE__)
 inline int g_planes_form = 0;   // host only: 0 by count, 1 plane by plane, 2 transpose
@@ -756,8 +757,14 @@ LSC_HD void decode_block_u32(const uint32_t *in, uint32_t hdr, const uint32_t bi
     if (hdr & 0x80u) {
         for (uint32_t g = 0; 8 * g < rate; g++) {
             uint32_t CP[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            const uint32_t end = rate < 8 * g + 8 ? rate : 8 * g + 8;
-            for (uint32_t i = 8 * g; i < end; i++) columns_add_plane(CP, in[i], i - 8 * g);
+            const uint32_t n = rate - 8 * g < 8u ? rate - 8 * g : 8u;   // planes of this group
+            if (planes_at_once(n, kPlanesDecode)) {
+#pragma unroll
+                for (uint32_t i = 0; i < 8; i++) CP[i] = i < n ? in[8 * g + i] : 0u;
+                columns_of_planes8(CP);
+            } else {
+                for (uint32_t i = 0; i < n; i++) columns_add_plane(CP, in[8 * g + i], i);
+            }
 #pragma unroll
             for (int u = 0; u < 8; u++)
 #pragma unroll
@@ -777,15 +784,39 @@ LSC_HD void decode_block_u32(const uint32_t *in, uint32_t hdr, const uint32_t bi
         uint32_t CL[16], CH[16];
 #pragma unroll
         for (int j = 0; j < 16; j++) { CL[j] = 0; CH[j] = 0; }
-        const uint32_t e1 = rate < 16 * h + 8 ? rate : 16 * h + 8;
-        const uint32_t e2 = rate < 16 * h + 16 ? rate : 16 * h + 16;
-        for (uint32_t i = 16 * h; i < e1; i++) {
-            columns_add_plane(CL, in[2 * i], i - 16 * h);
-            columns_add_plane(CL + 8, in[2 * i + 1], i - 16 * h);
+        // planes 16h .. 16h+7 (low bytes) and 16h+8 .. 16h+15 (high bytes)
+        const uint32_t nl = rate - 16 * h < 8u ? rate - 16 * h : 8u;
+        const uint32_t nh = rate > 16 * h + 8 ? (rate - 16 * h - 8 < 8u ? rate - 16 * h - 8 : 8u) : 0u;
+        const uint32_t *inl = in + 32 * h, *inh = in + 32 * h + 16;
+        if (planes_at_once(nl, kPlanesDecode)) {
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++) {
+                CL[i] = i < nl ? inl[2 * i] : 0u;
+                CL[8 + i] = i < nl ? inl[2 * i + 1] : 0u;
+            }
+            columns_of_planes8(CL);
+            columns_of_planes8(CL + 8);
+        } else {
+            for (uint32_t i = 0; i < nl; i++) {
+                columns_add_plane(CL, inl[2 * i], i);
+                columns_add_plane(CL + 8, inl[2 * i + 1], i);
+            }
         }
-        for (uint32_t i = 16 * h + 8; i < e2; i++) {
-            columns_add_plane(CH, in[2 * i], i - 16 * h - 8);
-            columns_add_plane(CH + 8, in[2 * i + 1], i - 16 * h - 8);
+        if (nh) {
+            if (planes_at_once(nh, kPlanesDecode)) {
+#pragma unroll
+                for (uint32_t i = 0; i < 8; i++) {
+                    CH[i] = i < nh ? inh[2 * i] : 0u;
+                    CH[8 + i] = i < nh ? inh[2 * i + 1] : 0u;
+                }
+                columns_of_planes8(CH);
+                columns_of_planes8(CH + 8);
+            } else {
+                for (uint32_t i = 0; i < nh; i++) {
+                    columns_add_plane(CH, inh[2 * i], i);
+                    columns_add_plane(CH + 8, inh[2 * i + 1], i);
+                }
+            }
         }
         rows_from_columns(CL, CH, D);
 #pragma unroll

@@ -175,11 +175,13 @@ def _blocks_u32(rng, n):
     return out
 
 
+@pytest.mark.parametrize("form", [0, 1, 2], ids=["by_count", "per_plane", "transpose"])
 @pytest.mark.parametrize("mode", sorted(MODES))
-def test_encode_decode_u32_vs_oracle(block_host, orc, mode):
+def test_encode_decode_u32_vs_oracle(block_host, orc, mode, form):
     """uint32 blocks: header, payload and decoded values equal the oracle's (rate-32 blocks are flagged)."""
     bins, p = MODES[mode]
     rng = np.random.default_rng(7 + sum(map(ord, mode)))
+    block_host.bh_set_planes_form(form)
     for bins_ in (bins, (1000, 5000, 70000, 1 << 20)):
         b = np.array(bins_, np.uint32)
         for v in _blocks_u32(rng, 1500):
