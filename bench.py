@@ -74,6 +74,13 @@ def timed_loop(torch, fn, steps):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    # torch creates a HIP event at its first record; a creation that stalls the
+    # host while the stream has run dry would land between an event pair, so
+    # every event is created (recorded once) before the timed launches
+    for a, b in evs:
+        a.record(stream)
+        b.record(stream)
+    torch.cuda.synchronize()
     gpu_busy(torch)
     for a, b in evs:
         a.record(stream)
