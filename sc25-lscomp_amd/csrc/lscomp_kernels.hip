@@ -207,34 +207,30 @@ __device__ __forceinline__ void store_block(const Item &it, uint32_t bx, uint32_
     const uint32_t ny = it.y - y0 < 8u ? it.y - y0 : 8u;
     const uint32_t nz = it.z - z0 < 8u ? it.z - z0 : 8u;
     if (it.align != ALIGN2) {
-        // A whole row stores its first 12 bytes, a block at the end of a row
-        // its first nz/2 - 1 dwords; both then store their last dword with one
-        // shared instruction.  Timed on the CSSI volume, this was 1.5-2.5 %
-        // faster than one 16-byte store per row (lossless decode 0.829 ->
-        // 0.819 ms, lossy 0.748 -> 0.729), and 4 % faster than 8 16-byte
-        // stores with no per-row branch for whole blocks.
+        // z is even here, so nz is 2, 4, 6 or 8.  A row of 6 or 8 elements
+        // stores its first 12 bytes, one of 2 or 4 its first dword; rows of 8
+        // and of 4 then store their last dword with one shared instruction.
+        // A wave that holds the block at the end of a row (nz 6 when z mod 8
+        // is 6) issues the same two stores as the others.  Timed on the CSSI
+        // volume, the 12 + 4-byte form was 1.5-2.5 % faster than one 16-byte
+        // store per row (lossless decode 0.829 -> 0.819 ms, lossy 0.748 ->
+        // 0.729), and 4 % faster than 8 16-byte stores with no per-row branch
+        // for whole blocks.
         typedef uint32_t u32x3_a4 __attribute__((ext_vector_type(3), __aligned__(4)));
+        const bool wide = nz >= 6u, tail = nz == 8u || nz == 4u;
+        const uint32_t tail_at = nz == 8u ? 3u : 1u;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             if ((uint32_t)r >= ny) break;
             uint32_t *q32 = (uint32_t *)(p + (uint64_t)r * it.z);   // dword aligned
             const uint32_t d0 = D[4 * r], d1 = D[4 * r + 1], d2 = D[4 * r + 2], d3 = D[4 * r + 3];
-            uint32_t tail, tail_at;   // the row's last dword and its index
-            bool has_tail;
-            if (nz == 8u) {
+            if (wide) {
                 const u32x3_a4 t = {d0, d1, d2};
                 *(u32x3_a4 *)q32 = t;
-                tail = d3;
-                tail_at = 3;
-                has_tail = true;
             } else {
                 q32[0] = d0;
-                if (nz > 2u) q32[1] = d1;
-                tail = d2;
-                tail_at = 2;
-                has_tail = nz > 4u;
             }
-            if (has_tail) q32[tail_at] = tail;
+            if (tail) q32[tail_at] = nz == 8u ? d3 : d1;
         }
     } else {
         // Rows only 2-byte aligned: a row at a 4-byte boundary stores 16 bytes, a
@@ -553,7 +549,7 @@ struct TileShared {
     uint32_t tile, next, epoch, ntot;
     uint32_t next2[2];             // compression: next tile id, by iteration parity
     uint32_t cls;                  // the workgroup's tile-id class
-    uint32_t tagg[64];             // decode offset phase: payload words of a chunk's tiles
+    uint32_t tagg[128];            // decode offset phase: payload words of a chunk's tiles
     uint32_t chunk, chunk_agg;     // decode offset phase: chunk id, chunk total
 };
 
@@ -580,9 +576,9 @@ enum : uint32_t { LB_RETRY = 0, LB_NEXT = 1, LB_DONE = 2 };
 // window positions each; waves past the window report nothing) go to LDS
 // buffer `par`: nearest inclusive and empty
 // positions and the sum of the values up to the nearest inclusive one (all
-// values without one).  An aggregate is at most 2^20 words (a status word
-// covers at most 64 tiles of at most 512 blocks of at most 32 words, or 256
-// blocks of at most 62), so the aggregate lanes of a slice are summed in 32
+// values without one).  An aggregate is at most 2^21 words (a status word
+// covers at most 128 tiles of at most 512 blocks of at most 32 words, or 256
+// blocks of at most 62), so the 64 aggregate lanes of a slice are summed in 32
 // bits and the inclusive lane's value is added in 64.  No barrier.
 template <int TILE>
 __device__ __forceinline__ void lb_slices(const uint64_t w[lb_per(TILE)], uint32_t epoch, uint32_t kWin,
@@ -748,9 +744,10 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
 // where the loads it covers have returned) and is converted after the barrier.
 struct TileQ {
     uint32_t cls, nc;
-    __device__ __forceinline__ uint32_t take(Ctrl *ctrl) const
+    // the first of n consecutive raw values of this class's counter
+    __device__ __forceinline__ uint32_t take(Ctrl *ctrl, uint32_t n = 1u) const
     {
-        return __hip_atomic_fetch_add(&ctrl->ctr[cls].v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __hip_atomic_fetch_add(&ctrl->ctr[cls].v, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ __forceinline__ uint32_t id(uint32_t k) const { return k * nc + cls; }
 };
@@ -1126,41 +1123,51 @@ __device__ __forceinline__ uint64_t tile_offset(const uint64_t *tile_off, uint32
 }
 
 // First phase of a single-volume decode launch: the payload offset of every
-// tile, from the header bytes alone (1/128 of the volume).  Chunks of 64 tiles
-// come from their own counter, so every chunk is taken before any workgroup
-// starts decoding.  TILE/64 threads sum 4 16-byte header pieces each of one
-// tile (each load instruction of a wave reads TILE/4 contiguous header bytes
-// of each of 4096/TILE tiles), the tile sums of a chunk are scanned, chunks
-// are chained by the look-back (one status word per chunk), and tile_off[t] =
-// lb_word(epoch, LB_INCL, offset of tile t).  Called by every thread;
-// barriers inside.
+// tile, from the header bytes alone (1/128 of the volume).  Chunks of 128
+// tiles come from their own counter, so every chunk is taken before any
+// workgroup starts decoding.  TILE/128 threads sum 8 16-byte header pieces
+// each of one tile (each load instruction of a wave reads 64 contiguous header
+// bytes of each of 64 * 128 / TILE tiles), the tile sums of a chunk are
+// scanned (one wave, two tiles per lane), chunks are chained by the look-back
+// (one status word per chunk), and tile_off[t] = lb_word(epoch, LB_INCL,
+// offset of tile t).  A 600 x 1813 x 1558 volume at tile 512 has 406 chunks,
+// fewer than the 512 workgroups of a full grid.  Called by every thread;
+// barriers inside.  Returns the workgroup's first chunk ticket (uniform): its
+// place in the start order, which the decode uses as its class ticket instead
+// of a second launch-wide atomic.  When the grid covers every chunk in the
+// first round no workgroup takes a second ticket: each of these start-up
+// atomics goes to one address from every workgroup at once.
 template <int TILE, class SH>
-__device__ __forceinline__ void offset_phase(const Item &it, Ctrl *ctrl, uint64_t *status, uint64_t *tile_off,
-                                             uint32_t epoch, uint32_t win, SH &sh)
+__device__ __forceinline__ uint32_t offset_phase(const Item &it, Ctrl *ctrl, uint64_t *status, uint64_t *tile_off,
+                                                 uint32_t epoch, uint32_t win, SH &sh)
 {
-    constexpr uint32_t kPer = TILE / 64;   // threads per tile; 4 pieces each cover its TILE header bytes
+    constexpr uint32_t kChunk = 128;          // tiles per chunk
+    constexpr uint32_t kPer = TILE / kChunk;  // threads per tile; 8 pieces each cover its TILE header bytes
+    static_assert(kPer >= 1 && kPer * 8 * 16 == TILE, "offset phase: tile of 128, 256 or 512 blocks");
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t nchunks = (it.ntiles + 63) / 64;
+    const uint32_t nchunks = (it.ntiles + kChunk - 1) / kChunk;
     if (tid == 0) sh.chunk = __hip_atomic_fetch_add(&ctrl->chunk.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
+    const uint32_t ticket = sh.chunk;
     for (;;) {
         const uint32_t chunk = sh.chunk;
         if (chunk >= nchunks) break;
         const uint32_t j = tid / kPer, q = tid % kPer;
-        const uint64_t tbase = ((uint64_t)chunk * 64 + j) * TILE;   // the tile's first header byte
+        const uint64_t tbase = ((uint64_t)chunk * kChunk + j) * TILE;   // the tile's first header byte
         uint32_t sum = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++) sum += words16(it, tbase + 16ull * (q + k * kPer));
+        for (uint32_t k = 0; k < 8; k++) sum += words16(it, tbase + 16ull * (q + k * kPer));
 #pragma unroll
         for (uint32_t d = 1; d < kPer; d <<= 1) sum += __shfl_xor(sum, (int)d, 64);
         if (q == 0) sh.tagg[j] = sum;
         __syncthreads();
-        uint32_t incl = 0;
+        uint32_t incl = 0, incl2 = 0;   // inclusive sums of tiles lane and 64 + lane
         if (tid < 64) {
             incl = wave_incl_scan(sh.tagg[lane], lane);
+            incl2 = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63) + wave_incl_scan(sh.tagg[64 + lane], lane);
             if (lane == 63) {
-                sh.chunk_agg = incl;
-                st_relaxed(status + chunk, lb_word(epoch, chunk == 0 ? LB_INCL : LB_AGG, incl));
+                sh.chunk_agg = incl2;
+                st_relaxed(status + chunk, lb_word(epoch, chunk == 0 ? LB_INCL : LB_AGG, incl2));
             }
         }
         __syncthreads();
@@ -1170,14 +1177,24 @@ __device__ __forceinline__ void offset_phase(const Item &it, Ctrl *ctrl, uint64_
             if (tid == 0) st_relaxed(status + chunk, lb_word(epoch, LB_INCL, excl + sh.chunk_agg));
         }
         if (tid < 64) {
-            const uint32_t t = chunk * 64 + lane;
+            const uint32_t t = chunk * kChunk + lane;
             if (t < it.ntiles) st_relaxed(tile_off + t, lb_word(epoch, LB_INCL, excl + incl - sh.tagg[lane]));
+            if (t + 64 < it.ntiles)
+                st_relaxed(tile_off + t + 64, lb_word(epoch, LB_INCL, excl + incl2 - sh.tagg[64 + lane]));
         }
         __syncthreads();
+        if (nchunks <= gridDim.x) break;   // tickets 0 .. gridDim.x - 1 went out in the first round
         if (tid == 0)
             sh.chunk = __hip_atomic_fetch_add(&ctrl->chunk.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
     }
+    return ticket;
+}
+
+// Thread 0: the class of a workgroup whose start ticket is known.
+__device__ __forceinline__ uint32_t class_of(uint32_t ticket, Sched sc)
+{
+    return sc.nc <= 1 ? 0u : ticket % sc.nc;
 }
 
 // kOff: the tile offsets are precomputed in tile_off (by the launch's offset
@@ -1198,15 +1215,26 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     }
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
-    if (!kTable && phase) offset_phase<TILE>(single, ctrl, status, tile_off, epoch, sc.win, sh);
+    uint32_t ticket = 0;
+    if (!kTable && phase) ticket = offset_phase<TILE>(single, ctrl, status, tile_off, epoch, sc.win, sh);
     if (tid == 0) {
-        sh.cls = tile_class(ctrl, sc);
+        sh.cls = !kTable && phase ? class_of(ticket, sc) : tile_class(ctrl, sc);
         const TileQ q0{sh.cls, sc.nc};
-        sh.tile = q0.id(q0.take(ctrl));
-        if (kOff) sh.next = q0.take(ctrl);   // raw (TileQ)
+        // with offsets the first two ids come from one atomic (raw, converted
+        // after the barrier); with the look-back ids are taken one at a time
+        sh.tile = kOff ? q0.take(ctrl, 2u) : q0.id(q0.take(ctrl));
     }
     __syncthreads();
     const TileQ q = uniform_queue(sh.cls, sc);
+    if (kOff) {
+        const uint32_t k0 = sh.tile;
+        __syncthreads();   // every thread has read the raw value
+        if (tid == 0) {
+            sh.tile = q.id(k0);
+            sh.next = k0 + 1u;   // raw (TileQ)
+        }
+        __syncthreads();
+    }
 
     // With precomputed tile offsets (no look-back) the header byte and the
     // offset of the next tile are loaded while the current tile decodes, and
@@ -1342,18 +1370,18 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it_arg, Ctrl *c
     if (tid == 0) sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const uint32_t epoch = sh.epoch;
-    if (phase) offset_phase<TILE>(it, ctrl, status, tile_off, epoch, sc.win, sh);
-    uint32_t pend = 0;   // thread 0: the raw id of the tile two ahead of the next one (TileQ)
+    uint32_t ticket = 0;
+    if (phase) ticket = offset_phase<TILE>(it, ctrl, status, tile_off, epoch, sc.win, sh);
     if (tid == 0) {
-        sh.cls = tile_class(ctrl, sc);
+        sh.cls = phase ? class_of(ticket, sc) : tile_class(ctrl, sc);
         const TileQ q0{sh.cls, sc.nc};
-        sh.tile = q0.take(ctrl);
-        sh.next = q0.take(ctrl);
-        pend = q0.take(ctrl);
+        sh.tile = q0.take(ctrl, 3u);   // raw values k0 .. k0 + 2: this tile, the next, the one after
     }
     __syncthreads();
     const TileQ q = uniform_queue(sh.cls, sc);
-    uint32_t t = q.id(sh.tile), t1 = q.id(sh.next);
+    const uint32_t k0 = sh.tile;
+    uint32_t pend = k0 + 2u;   // thread 0: the raw id of the tile two ahead of the next one (TileQ)
+    uint32_t t = q.id(k0), t1 = q.id(k0 + 1u);
 
     // header byte of this thread's block in tile u, and u's payload byte offset
     const auto hdr_of = [&](const Item &it, uint32_t u) -> uint32_t {

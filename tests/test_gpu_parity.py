@@ -315,3 +315,34 @@ def test_decode_many_tiles_per_workgroup(gpu, orc, kind):
             check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
     finally:
         gpu.set_launch_shape(gpu.DECOMPRESS, 0, 0)
+
+
+@pytest.mark.parametrize("mode", ["lossless", "b3-5-10-15_p0.5"])
+def test_decode_offset_chunks_beyond_the_grid(gpu, orc, mode):
+    """The decode's offset phase takes chunks of 128 tiles; at tile 128 with one
+    workgroup per CU a 280 x 1024 x 1024 volume (4.6 M blocks, 35840 tiles, 280
+    chunks) has more chunks than a 256-CU grid has workgroups, so workgroups take
+    further chunks after their first.  Lossless decodes equal the input, lossy
+    ones the oracle's decode of the same stream, and the stream equals the
+    default shape's."""
+    import torch
+    bins, p = MODES[mode]
+    dims = (280, 1024, 1024)
+    vol = synth("cssi", dims, seed=47)
+    d = to_dev(vol)
+    ref_out, ref_n = gpu.compress(d, dims, bins, p)
+    try:
+        for k in (gpu.COMPRESS, gpu.DECOMPRESS):
+            gpu.set_launch_shape(k, 128, 1)
+        out, n = gpu.compress(d, dims, bins, p)
+        assert n == ref_n and bool(torch.equal(out[:n], ref_out[:ref_n]))
+        dec = gpu.decompress(out, n, dims, bins)
+        if tuple(bins) == (1, 1, 1, 1):
+            assert bool(torch.equal(dec, d))
+        else:
+            _, refdec = orc.decompress(out[:n].cpu().numpy(), dims, bins)
+            assert np.array_equal(dec.cpu().numpy().view(np.uint16), refdec)
+    finally:
+        for k in (gpu.COMPRESS, gpu.DECOMPRESS):
+            gpu.set_launch_shape(k, 0, 0)
+    torch.cuda.synchronize()
