@@ -780,14 +780,21 @@ __device__ __forceinline__ void finish_launch(Ctrl *ctrl, uint32_t epoch)
 // counter, so that wait would also cover every access issued before it.  The
 // raw value goes to LDS where the old tile was written anyway (the wait falls
 // where the loads it covers have returned) and is converted after the barrier.
+//
+// Compression with several counters hands out the first gridDim.x tiles by
+// the start ticket itself (the workgroup with ticket j starts with tile j,
+// whose class j mod nc is its own), so one atomic gives both; the class
+// counters then start after those tiles (`base`: the number of tiles of the
+// class below gridDim.x).  The argument above holds unchanged: the first
+// tiles are held by running workgroups in start order.
 struct TileQ {
-    uint32_t cls, nc;
+    uint32_t cls, nc, base;   // base: ids of this class handed out by the start tickets
     // the first of n consecutive raw values of this class's counter
     __device__ __forceinline__ uint32_t take(Ctrl *ctrl, uint32_t n = 1u) const
     {
         return __hip_atomic_fetch_add(&ctrl->ctr[cls].v, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __device__ __forceinline__ uint32_t id(uint32_t k) const { return k * nc + cls; }
+    __device__ __forceinline__ uint32_t id(uint32_t k) const { return (k + base) * nc + cls; }
 };
 
 // Thread 0: the workgroup's class (one ticket atomic when there are several
@@ -799,9 +806,10 @@ __device__ __forceinline__ uint32_t tile_class(Ctrl *ctrl, Sched sc)
     return __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) % sc.nc;
 }
 
-__device__ __forceinline__ TileQ uniform_queue(uint32_t cls, Sched sc)
+__device__ __forceinline__ TileQ uniform_queue(uint32_t cls, Sched sc, uint32_t base = 0u)
 {
-    return TileQ{(uint32_t)__builtin_amdgcn_readfirstlane(cls), (uint32_t)__builtin_amdgcn_readfirstlane(sc.nc)};
+    return TileQ{(uint32_t)__builtin_amdgcn_readfirstlane(cls), (uint32_t)__builtin_amdgcn_readfirstlane(sc.nc),
+                 (uint32_t)__builtin_amdgcn_readfirstlane(base)};
 }
 
 // The single item, read again from the kernel argument segment where it is
@@ -935,13 +943,16 @@ __device__ __forceinline__ void compress_tiles(const Item single, const Item *__
     if (tid == 0) {
         sh.epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.ntot = d_total_tiles ? *d_total_tiles : total_tiles;
-        sh.cls = tile_class(ctrl, sc);
-        sh.next = TileQ{sh.cls, sc.nc}.take(ctrl);
+        // the first tile: with one counter its first value, with several the
+        // start ticket (TileQ); raw, turned into the class after the barrier
+        sh.next = sc.nc <= 1 ? TileQ{0u, 1u, 0u}.take(ctrl)
+                             : __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
-    const TileQ q = uniform_queue(sh.cls, sc);
-    const uint32_t n0 = q.id(sh.next);
+    const uint32_t n0 = sh.next;
+    const uint32_t cls = sc.nc <= 1 ? 0u : n0 % sc.nc;
+    const TileQ q = uniform_queue(cls, sc, sc.nc <= 1 ? 0u : (gridDim.x - cls + sc.nc - 1u) / sc.nc);
 
     uint32_t np = 0, hb = 0;   // pending tiles (p0 oldest), header buffer of the next tile
     PendTile p0 = {0, 0, 0, 0, 0, 0}, p1 = {0, 0, 0, 0, 0, 0};
