@@ -16,6 +16,10 @@ served form (512^2..2048^2 CSSI frames, half lossless and half lossy, under a
 64 GB budget: the device-resident loop at the offered 20k frames/s and at
 saturation, beside the host-scheduled engine and plain per-frame launches).
 
+Each mode also carries the end-to-end time of one blocking call (the stream size
+or status read back on the host, as the reference's CLI times its calls) and that
+rate in the reference's unit (input MiB per ms).
+
 Prints ONE JSON line.  `--gpus` above 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -91,6 +95,17 @@ def timed_loop(torch, fn, steps):
     return wall, sum(per) / len(per), per
 
 
+def e2e_ms(torch, fn, steps):
+    """Mean wall-clock ms of `steps` blocking calls (each returns after its stream
+    size or status has reached the host)."""
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
 def profiled_traffic(dims):
     """HBM bytes per compression launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE of the compress kernel, scaled by the same counters
@@ -131,6 +146,14 @@ def time_codec(torch, lscomp, d_in, dims, bins, pool, d_out, d_dec, d_size, args
         "compress_hbm_frac": (in_bytes + s_bytes) / (c_ev * 1e-3) / HBM_PEAK,
         "decompress_hbm_frac": (in_bytes + s_bytes) / (d_ev * 1e-3) / HBM_PEAK,
     }
+    # end to end per blocking call, as the reference's CLI times it (lsCOMP_xpcs.cpp:144-174:
+    # the call returns with the stream size on the host), and in the reference's unit
+    # (input MiB per ms, i.e. 1.048576e9 B/s)
+    c_e2e = e2e_ms(torch, lambda: lscomp.compress(d_in, dims, bins, pool, out=d_out), args.steps)
+    d_e2e = e2e_ms(torch, lambda: lscomp.decompress(d_out, s_bytes, dims, bins, out=d_dec), args.steps)
+    res.update({"compress_e2e_ms": c_e2e, "decompress_e2e_ms": d_e2e,
+                "compress_e2e_ref_unit_GBps": in_bytes / 1048576.0 / c_e2e,
+                "decompress_e2e_ref_unit_GBps": in_bytes / 1048576.0 / d_e2e})
     if check_lossless:
         ok = ok and bool(torch.equal(d_dec, d_in))
     else:
@@ -215,6 +238,33 @@ def cpu_baseline(vol, bins, pooling, frames_all=600, frames_one=96, reps=5):
     one = np.ascontiguousarray(vol[:frames_one])
     dt1 = median_pass(one, False)
     threads = orc.max_threads()
+
+    # the other configurations, each on a bounded sample (GB/s of input, decimal)
+    def config(v, b, p, n_all, n_one):
+        a_ = np.ascontiguousarray(v[:n_all])
+        o_ = np.ascontiguousarray(v[:n_one])
+        ta = median_pass_s(a_, b, p, True)
+        to = median_pass_s(o_, b, p, False)
+        return {"all_threads_GBps": a_.nbytes / ta / 1e9, "one_thread_GBps": o_.nbytes / to / 1e9,
+                "frames_all_threads": int(a_.shape[0]), "frames_one_thread": int(o_.shape[0])}
+
+    def median_pass_s(v, b, p, omp):
+        orc.compress(v, b, p, omp=omp)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            orc.compress(v, b, p, omp=omp)
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    from lscomp import datagen
+    configs = {"cssi_600x1813x1558_b3-5-10-15_p0.5": config(vol, (3, 5, 10, 15), 0.5, frames_all, 48)}
+    small = datagen.volume("sparse", 1, SMALL_DIMS)
+    configs["small_16x512x512_lossless"] = config(small, (1, 1, 1, 1), 1.0, SMALL_DIMS[0], SMALL_DIMS[0])
+    xp = datagen.volume("xpcs", 3, (256,) + XPCS_DIMS[1:])   # the first 256 frames of the XPCS volume
+    configs["xpcs_512x1024x1024_lossless"] = config(xp, (1, 1, 1, 1), 1.0, 256, 64)
+    configs["xpcs_512x1024x1024_b3x4_p0.5"] = config(xp, (3, 3, 3, 3), 0.5, 256, 64)
+    del xp, small
     return {"value": sub.nbytes / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"lossless compression of the first {frames_all} frames ({sub.nbytes / 1e9:.3f} GB) of the "
                       f"same volume, OpenMP two-pass oracle on {threads} threads, 1 warm-up + median of {reps} "
@@ -222,6 +272,7 @@ def cpu_baseline(vol, bins, pooling, frames_all=600, frames_one=96, reps=5):
             "value_1_thread": one.nbytes / dt1 / 1e9,
             "sample_1_thread": f"serial oracle, first {frames_one} frames ({one.nbytes / 1e9:.3f} GB), "
                                f"1 warm-up + median of {reps} ({dt1:.3f} s)",
+            "configs": configs,
             "cpu_model": cpu_info(), "nproc": os.cpu_count(),
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
