@@ -1388,6 +1388,68 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     finish_launch(ctrl, epoch);
 }
 
+// Decompression of a single item of at most kSmallTiles tiles: one workgroup
+// per tile, which sums the payload sizes of every block before its tile from
+// their header bytes itself (at most 255 * TILE bytes, 16-byte pieces strided
+// over the workgroup, four loads in flight per thread).  No workgroup waits
+// for another and no counter or epoch is used, so the launch is a chain of
+// three memory round trips (header bytes, payload, row stores) instead of the
+// offset phase's chunk ticket, header pass, look-back and tile ids ahead of
+// the decode.  The header bytes before a tile are read once per later tile,
+// but the whole header section is at most 128 KB and stays in L2.
+constexpr uint32_t kSmallTiles = 256;
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_small_kernel(
+    const Item it, Ctrl *ctrl)
+{
+    __shared__ TileShared<TILE> sh;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t t = blockIdx.x;
+    const uint32_t b = t * (uint32_t)TILE + tid;
+    const bool valid = b < it.nblocks;
+    const uint32_t hdr = valid ? (uint32_t)it.s_in[b] : 0u;
+    // payload words before this tile: at most 255 tiles of 512 blocks of 32
+    // words, so 32 bits hold every partial sum
+    uint32_t before = 0;
+    const uint32_t npieces = t * ((uint32_t)TILE / 16u);
+    for (uint32_t i = tid; i < npieces; i += 4u * TILE) {
+        uint32_t w[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t j = i + k * (uint32_t)TILE;
+            w[k] = j < npieces ? words16(it, 16ull * j) : 0u;
+        }
+        before += (w[0] + w[1]) + (w[2] + w[3]);
+    }
+    const uint32_t wtot = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(before, lane), 63);
+    if (lane == 0) sh.wsum[1][wave] = wtot;
+    const uint32_t words = header_payload_bytes(hdr) >> 2;
+    uint32_t agg;
+    const uint32_t off = tile_scan<TILE>(words, tid, sh.wsum[0], &agg);   // its barrier also covers wsum[1]
+    uint32_t excl = 0;
+#pragma unroll
+    for (int w = 0; w < TILE / 64; w++) excl += sh.wsum[1][w];
+    const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
+    const bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
+    if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
+    __syncthreads();
+    if (valid) {
+        const bool bad = (hdr & 0x1fu) > 16u || !fits;
+        uint32_t D[32];
+        if (!bad && (hdr & 0x1fu)) {
+            decode_block(&sh.pay[((uint32_t)(g0 & 15) >> 2) + off], hdr, it.dec.b, D);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 32; i++) D[i] = 0;
+        }
+        uint32_t bx, by, bz;
+        block_coords(it, b, bx, by, bz);
+        store_block<true>(it, bx, by, bz, D);
+        if (bad) __hip_atomic_fetch_or(&ctrl->error, ERRF_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Decompression of a single item with precomputed tile offsets, pipelined so
 // that a tile's payload loads overlap the decoding of the tile before it.  The
 // iteration that decodes tile t (payload in the LDS ring at t_start):
@@ -1698,6 +1760,8 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
         if (items)
             hipLaunchKernelGGL((decompress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
                                cfg.total_tiles, cfg.d_total_tiles, ctrl, status, (uint64_t *)nullptr, sc, 0u);
+        else if (single.ntiles <= kSmallTiles)
+            hipLaunchKernelGGL((decompress_small_kernel<TILE>), dim3(single.ntiles), block, 0, stream, single, ctrl);
         else
         {
             const uint32_t phase = cfg.tile_off && cfg.off_mode == OFF_PHASE ? 1u : 0u;

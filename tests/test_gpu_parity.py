@@ -346,3 +346,14 @@ def test_decode_offset_chunks_beyond_the_grid(gpu, orc, mode):
         for k in (gpu.COMPRESS, gpu.DECOMPRESS):
             gpu.set_launch_shape(k, 0, 0)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("shape", [(1, 4096, 2048), (1, 4096, 2056), (16, 512, 512), (1, 8, 9)],
+                         ids=["256tiles", "257tiles", "128tiles", "1tile"])
+@pytest.mark.parametrize("mode", ["lossless", "b3-5-10-15_p0.5"])
+def test_small_volume_decode(gpu, orc, shape, mode):
+    """Volumes of at most 256 tiles decode with one workgroup per tile that sums
+    the header bytes before its tile itself; 257 tiles take the launch with the
+    offset phase.  Both sides of the limit equal the oracle's stream and decode."""
+    bins, p = MODES[mode]
+    check_case(gpu, orc, synth("cssi", shape, seed=61 + shape[1] % 7), bins, p)
