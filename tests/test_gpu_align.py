@@ -16,8 +16,11 @@ pytestmark = pytest.mark.gpu
 OFFSETS = (0, 4, 8, 12)
 MODES = {"lossless": ((1, 1, 1, 1), 1.0), "lossy": ((3, 5, 10, 15), 0.5)}
 GUARD = 0xC3
-# several tiles (the look-back and the payload ring cross tiles), partial blocks
-SHAPES = [(3, 77, 130), (5, 300, 264)]
+# several tiles (the look-back and the payload ring cross tiles), partial blocks;
+# the last has more than 256 tiles, so its decode reads the stream through the
+# offset phase and the pipelined (lossless) or sparse (lossy) decoder rather
+# than with one workgroup per tile
+SHAPES = [(3, 77, 130), (5, 300, 264), (10, 1000, 1030)]
 
 
 def _window(n, off):

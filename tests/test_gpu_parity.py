@@ -177,13 +177,17 @@ def test_decode_oracle_stream(gpu, orc):
 @pytest.mark.parametrize("tile", [128, 256, 512])
 @pytest.mark.parametrize("wg_per_cu", [0, 1])
 def test_launch_shapes_give_identical_bytes(gpu, orc, tile, wg_per_cu):
-    """The stream does not depend on the launch shape (tile size, persistent grid size)."""
-    vol = synth("cssi", (6, 300, 1558), seed=13)
+    """The stream does not depend on the launch shape (tile size, persistent grid size).
+    The 12 x 600 x 1558 volume (175,500 blocks) has more than 256 tiles at every
+    tile size, so it also decodes through the offset-phase launch; the smaller
+    one decodes with one workgroup per tile where it has at most 256."""
     try:
         for d in (gpu.COMPRESS, gpu.DECOMPRESS):
             gpu.set_launch_shape(d, tile, wg_per_cu)
-        check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
-        check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
+        for shape in ((6, 300, 1558), (12, 600, 1558)):
+            vol = synth("cssi", shape, seed=13)
+            check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+            check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
     finally:
         for d in (gpu.COMPRESS, gpu.DECOMPRESS):
             gpu.set_launch_shape(d, 0, 0)
@@ -217,15 +221,18 @@ def test_autotune_applies_a_shape_and_keeps_bytes(gpu, orc):
 
 @pytest.mark.parametrize("shape", [(128, 0, 128, 1), (256, 1, 256, 16), (512, 2, 512, 8), (512, 0, 64, 3)])
 def test_every_window_and_counter_count_gives_identical_bytes(gpu, orc, shape):
-    """The stream does not depend on the look-back window or the number of tile-id counters."""
-    vol = synth("cssi", (5, 300, 1558), seed=19)
+    """The stream does not depend on the look-back window or the number of tile-id
+    counters (the 12 x 600 x 1558 volume decodes through the offset-phase launch
+    at every tile size)."""
     tile, w, win, nc = shape
     saved = [gpu.get_launch_shape_ex(k) for k in (gpu.COMPRESS, gpu.DECOMPRESS)]
     try:
         for k in (gpu.COMPRESS, gpu.DECOMPRESS):
             gpu.set_launch_shape_ex(k, tile, w, win, nc)
-        check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
-        check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
+        for shape3 in ((5, 300, 1558), (12, 600, 1558)):
+            vol = synth("cssi", shape3, seed=19)
+            check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
+            check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
     finally:
         for k, sh in zip((gpu.COMPRESS, gpu.DECOMPRESS), saved):
             gpu.set_launch_shape_ex(k, **sh)
@@ -278,8 +285,10 @@ def test_decode_with_and_without_prepass(gpu, orc, prepass):
     the decode launch (1) or a separate pre-pass kernel (2): same volume."""
     old = gpu.set_decode_prepass(prepass)
     try:
+        # the last two have more than 256 tiles (at most 256 decode with one
+        # workgroup per tile in every mode)
         for shape, kind in (((3, 300, 1558), "cssi"), ((1, 8, 9), "uniform"), ((64, 128, 128), "xpcs"),
-                            ((2, 2048, 2056), "mixed")):
+                            ((2, 2048, 2056), "mixed"), ((12, 600, 1558), "cssi")):
             vol = synth(kind, shape, seed=31)
             check_case(gpu, orc, vol, (1, 1, 1, 1), 1.0)
             check_case(gpu, orc, vol, (3, 5, 10, 15), 0.5)
