@@ -280,7 +280,8 @@ def cpu_baseline(vol, bins, pooling, frames_all=600, frames_one=96, reps=5):
 def bench_batched(torch, lscomp, datagen, np, steps):
     """512 square frames (sides drawn from 256..2048, XPCS-style, seed 4): one persistent
     plan launch (shape autotuned on the batch, replayed from a hipGraph) vs 512 plain
-    per-frame launches; identical bytes."""
+    per-frame launches; identical bytes.  Then the same streams decoded both ways (a
+    decompression plan in one graph launch vs 512 plain decodes), each equal to the input."""
     rng = np.random.default_rng(4)
     sides = rng.choice([256, 512, 768, 1024, 1536, 2048], size=512)
     frames = [torch.from_numpy(datagen.volume("xpcs", 4, (1, int(s), int(s)), first_frame=i).view(np.int16)).cuda()
@@ -318,13 +319,38 @@ def bench_batched(torch, lscomp, datagen, np, steps):
     same = bsz == psz and all(torch.equal(a[:n], b[:n]) for a, b, n in zip(out_plain, out_batch, bsz))
     ok = plan.status() == lscomp.OK and lscomp.stream_status() == lscomp.OK
     plan.close()
+    del out_plain
+
+    # the same 512 streams decoded: one plan launch (graph) vs 512 plain launches
+    dec_plain = [torch.empty_like(f) for f in frames]
+    dec_batch = [torch.empty_like(f) for f in frames]
+
+    def dplain():
+        for i, d in enumerate(dims):
+            lscomp.decompress_async(out_batch[i], bsz[i], d, bins, dec_plain[i])
+
+    dplan = lscomp.Plan(lscomp.DECOMPRESS, len(frames), nblocks)
+    dplan.set_items([(o, d, bins, pool, sb, n) for o, d, sb, n in zip(dec_batch, dims, out_batch, bsz)])
+    dbatch = lambda: dplan.launch(graph=True)
+    for fn in (dplain, dbatch):
+        fn()
+    torch.cuda.synchronize()
+    _, dplain_ms, _ = timed_loop(torch, dplain, steps)
+    _, dbatch_ms, _ = timed_loop(torch, dbatch, steps)
+    dsame = all(torch.equal(a, f) and torch.equal(b, f) for a, b, f in zip(dec_plain, dec_batch, frames))
+    dok = dplan.status() == lscomp.OK and lscomp.stream_status() == lscomp.OK
+    dplan.close()
     return {"frames": len(frames), "input_bytes": in_bytes,
             "plain_launches_ms": plain_ms, "plain_frames_per_s": len(frames) / (plain_ms * 1e-3),
             "batched_graph_ms": batch_ms, "batched_frames_per_s": len(frames) / (batch_ms * 1e-3),
             "batched_GBps": in_bytes / (batch_ms * 1e-3) / 1e9, "speedup": plain_ms / batch_ms,
             "tuned_shape": tuned_shape, "default_shape": default_shape, "default_shape_ms": default_ms,
             "tuner_shapes_measured": len(table),
-            "identical_streams": bool(same), "verified": bool(ok)}
+            "identical_streams": bool(same), "verified": bool(ok),
+            "decode_plain_launches_ms": dplain_ms, "decode_plain_frames_per_s": len(frames) / (dplain_ms * 1e-3),
+            "decode_batched_graph_ms": dbatch_ms, "decode_batched_frames_per_s": len(frames) / (dbatch_ms * 1e-3),
+            "decode_batched_GBps": in_bytes / (dbatch_ms * 1e-3) / 1e9, "decode_speedup": dplain_ms / dbatch_ms,
+            "decode_identical_to_input": bool(dsame), "decode_verified": bool(dok)}
 
 
 def bench_served(torch, lscomp, datagen, np, seconds, rate, n_sat):
