@@ -31,6 +31,7 @@ struct lscomp_plan_s {
     uint64_t *d_sizes = nullptr;  // per-item results
     Ctrl *ctrl = nullptr;
     uint64_t *status = nullptr;
+    uint64_t *tile_off = nullptr; // decompression: payload offset of every tile over the batch (offset phase)
     // host staging (pinned) for the table uploads, two buffers alternating
     uint8_t *h_stage[2] = {nullptr, nullptr};
     hipEvent_t stage_done[2] = {nullptr, nullptr};
@@ -65,6 +66,7 @@ void free_plan(lscomp_plan p)
     if (p->d_sizes) (void)hipFree(p->d_sizes);
     if (p->ctrl) (void)hipFree(p->ctrl);
     if (p->status) (void)hipFree(p->status);
+    if (p->tile_off) (void)hipFree(p->tile_off);
     delete p;
 }
 
@@ -72,6 +74,7 @@ int count_launch(lscomp_plan p, hipStream_t stream)
 {
     if (++p->launches >= kEpochClearPeriod) {
         int st = hip_fail(launch_fill_u64(p->status, p->max_tiles, 0, stream), "fill");
+        if (!st && p->tile_off) st = hip_fail(launch_fill_u64(p->tile_off, p->max_tiles, 0, stream), "fill");
         if (st) return st;
         p->launches = 1;
     }
@@ -85,6 +88,10 @@ LaunchCfg plan_cfg(lscomp_plan p)
     LaunchCfg c = make_cfg(p->max_tiles, p->direction == 0, true, p->shape);
     c.total_tiles = 0;
     c.d_total_tiles = p->d_total;
+    // decompression: every tile's offset from the header bytes first, so no
+    // tile waits for another and tile ids are taken ahead
+    c.tile_off = p->direction == 0 ? nullptr : p->tile_off;
+    c.off_mode = OFF_PHASE;
     return c;
 }
 
@@ -200,6 +207,8 @@ int lscomp_plan_create(int direction, uint32_t max_items, uint64_t max_blocks, l
               !hip_fail(hipMalloc((void **)&p->d_sizes, sizeof(uint64_t) * max_items), "hipMalloc") &&
               !hip_fail(hipMalloc((void **)&p->ctrl, sizeof(Ctrl)), "hipMalloc") &&
               !hip_fail(hipMalloc((void **)&p->status, sizeof(uint64_t) * p->max_tiles), "hipMalloc") &&
+              (direction == 0 ||
+               !hip_fail(hipMalloc((void **)&p->tile_off, sizeof(uint64_t) * p->max_tiles), "hipMalloc")) &&
               !hip_fail(hipHostMalloc((void **)&p->h_ctrl, sizeof(Ctrl), hipHostMallocDefault), "hipHostMalloc");
     for (int i = 0; ok && i < 2; i++)
         ok = !hip_fail(hipHostMalloc((void **)&p->h_stage[i], p->stage_bytes, hipHostMallocDefault), "hipHostMalloc") &&
@@ -208,6 +217,7 @@ int lscomp_plan_create(int direction, uint32_t max_items, uint64_t max_blocks, l
     // zero scratch and tables with a fill kernel, then wait (creation is not stream-ordered)
     ok = !hip_fail(launch_fill_u64((uint64_t *)p->ctrl, sizeof(Ctrl) / 8, 0, nullptr), "fill") &&
          !hip_fail(launch_fill_u64(p->status, p->max_tiles, 0, nullptr), "fill") &&
+         (!p->tile_off || !hip_fail(launch_fill_u64(p->tile_off, p->max_tiles, 0, nullptr), "fill")) &&
          !hip_fail(launch_fill_u64((uint64_t *)p->d_total, 1, 0, nullptr), "fill") &&
          !hip_fail(launch_fill_u64(p->d_sizes, max_items, 0, nullptr), "fill") &&
          !hip_fail(hipStreamSynchronize(nullptr), "hipStreamSynchronize");

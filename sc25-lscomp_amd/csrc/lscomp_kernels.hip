@@ -1186,15 +1186,21 @@ __device__ __forceinline__ uint64_t tile_offset(const uint64_t *tile_off, uint32
 // of a second launch-wide atomic.  When the grid covers every chunk in the
 // first round no workgroup takes a second ticket: each of these start-up
 // atomics goes to one address from every workgroup at once.
-template <int TILE, class SH>
-__device__ __forceinline__ uint32_t offset_phase(const Item &it, Ctrl *ctrl, uint64_t *status, uint64_t *tile_off,
+//
+// Batches (kTable): the tiles of every item in global tile order, each tile's
+// header bytes read from its own item; tile_off[t] is then the prefix over all
+// tiles before t, and a tile's offset within its item is tile_off[t] minus
+// tile_off of the item's first tile.
+template <int TILE, bool kTable = false, class SH>
+__device__ __forceinline__ uint32_t offset_phase(const Item &single, const Item *items, const uint32_t *tile_items,
+                                                 uint32_t ntot, Ctrl *ctrl, uint64_t *status, uint64_t *tile_off,
                                                  uint32_t epoch, uint32_t win, SH &sh)
 {
     constexpr uint32_t kChunk = 128;          // tiles per chunk
     constexpr uint32_t kPer = TILE / kChunk;  // threads per tile; 8 pieces each cover its TILE header bytes
     static_assert(kPer >= 1 && kPer * 8 * 16 == TILE, "offset phase: tile of 128, 256 or 512 blocks");
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t nchunks = (it.ntiles + kChunk - 1) / kChunk;
+    const uint32_t nchunks = (ntot + kChunk - 1) / kChunk;
     if (tid == 0) sh.chunk = __hip_atomic_fetch_add(&ctrl->chunk.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const uint32_t ticket = sh.chunk;
@@ -1202,10 +1208,15 @@ __device__ __forceinline__ uint32_t offset_phase(const Item &it, Ctrl *ctrl, uin
         const uint32_t chunk = sh.chunk;
         if (chunk >= nchunks) break;
         const uint32_t j = tid / kPer, q = tid % kPer;
-        const uint64_t tbase = ((uint64_t)chunk * kChunk + j) * TILE;   // the tile's first header byte
+        const uint32_t t = chunk * kChunk + j;   // this thread's tile
         uint32_t sum = 0;
+        if (t < ntot) {
+            // per lane: the 64 / kPer tiles of a wave may belong to different items
+            const Item &it = kTable ? items[tile_items[t]] : single;
+            const uint64_t tbase = (uint64_t)(t - it.first_tile) * TILE;   // the tile's first header byte
 #pragma unroll
-        for (uint32_t k = 0; k < 8; k++) sum += words16(it, tbase + 16ull * (q + k * kPer));
+            for (uint32_t k = 0; k < 8; k++) sum += words16(it, tbase + 16ull * (q + k * kPer));
+        }
 #pragma unroll
         for (uint32_t d = 1; d < kPer; d <<= 1) sum += __shfl_xor(sum, (int)d, 64);
         if (q == 0) sh.tagg[j] = sum;
@@ -1227,8 +1238,8 @@ __device__ __forceinline__ uint32_t offset_phase(const Item &it, Ctrl *ctrl, uin
         }
         if (tid < 64) {
             const uint32_t t = chunk * kChunk + lane;
-            if (t < it.ntiles) st_relaxed(tile_off + t, lb_word(epoch, LB_INCL, excl + incl - sh.tagg[lane]));
-            if (t + 64 < it.ntiles)
+            if (t < ntot) st_relaxed(tile_off + t, lb_word(epoch, LB_INCL, excl + incl - sh.tagg[lane]));
+            if (t + 64 < ntot)
                 st_relaxed(tile_off + t + 64, lb_word(epoch, LB_INCL, excl + incl2 - sh.tagg[64 + lane]));
         }
         __syncthreads();
@@ -1265,9 +1276,10 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     __syncthreads();
     const uint32_t epoch = sh.epoch, ntot = sh.ntot;
     uint32_t ticket = 0;
-    if (!kTable && phase) ticket = offset_phase<TILE>(single, ctrl, status, tile_off, epoch, sc.win, sh);
+    if (phase)
+        ticket = offset_phase<TILE, kTable>(single, items, tile_items, ntot, ctrl, status, tile_off, epoch, sc.win, sh);
     if (tid == 0) {
-        sh.cls = !kTable && phase ? class_of(ticket, sc) : tile_class(ctrl, sc);
+        sh.cls = phase ? class_of(ticket, sc) : tile_class(ctrl, sc);
         const TileQ q0{sh.cls, sc.nc};
         // with offsets the first two ids come from one atomic (raw, converted
         // after the barrier); with the look-back ids are taken one at a time
@@ -1298,11 +1310,13 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
     uint32_t *const hdr_slot = &sh.hdr[0][0] + tid;
     uint32_t hdr = 0;
     uint64_t w_next = 0;   // offset word of the next tile, checked when used
+    uint64_t wb_next = 0;  // batches: offset word of the next tile's item's first tile (its base)
     if (kOff && sh.tile < ntot) {
         const Item &it = tile_item<kTable>(single, items, tile_items, sh.tile);
         const uint32_t b = (sh.tile - it.first_tile) * (uint32_t)TILE + tid;
         hdr = b < it.nblocks ? (uint32_t)it.s_in[b] : 0u;
         w_next = off_word(tile_off, sh.tile);
+        if (kTable) wb_next = off_word(tile_off, it.first_tile);
     }
     if (kOff) *hdr_slot = hdr;
 
@@ -1318,8 +1332,10 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
         const bool valid = b < it.nblocks;
         hdr = kOff ? *hdr_slot : valid ? (uint32_t)it.s_in[b] : 0u;
 
-        // with precomputed tile offsets no look-back is needed
+        // with precomputed tile offsets no look-back is needed (in a batch the
+        // offsets run over every item's tiles: less the item's first tile's)
         uint64_t excl = kOff ? off_value(tile_off, tile, w_next, epoch, ctrl) : 0ull;
+        if (kOff && kTable) excl -= off_value(tile_off, it.first_tile, wb_next, epoch, ctrl);
         const uint32_t words = header_payload_bytes(hdr) >> 2;
         bool bad = (hdr & 0x1fu) > 16u;
 
@@ -1348,6 +1364,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
             const uint32_t nb = (ntile - nit.first_tile) * (uint32_t)TILE + tid;
             hdr_next = nb < nit.nblocks ? (uint32_t)nit.s_in[nb] : 0u;
             if (kOff) w_next = off_word(tile_off, ntile);
+            if (kOff && kTable) wb_next = off_word(tile_off, nit.first_tile);
         }
         if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
         if (kOff) *hdr_slot = hdr_next;   // waits for the byte here, before this tile's stores
@@ -1482,7 +1499,8 @@ __device__ __forceinline__ void decompress_pipe_tiles(const Item it_arg, Ctrl *c
     __syncthreads();
     const uint32_t epoch = sh.epoch;
     uint32_t ticket = 0;
-    if (phase) ticket = offset_phase<TILE>(it, ctrl, status, tile_off, epoch, sc.win, sh);
+    if (phase)
+        ticket = offset_phase<TILE>(it, nullptr, nullptr, it.ntiles, ctrl, status, tile_off, epoch, sc.win, sh);
     if (tid == 0) {
         sh.cls = phase ? class_of(ticket, sc) : tile_class(ctrl, sc);
         const TileQ q0{sh.cls, sc.nc};
@@ -1757,7 +1775,10 @@ static hipError_t launch_t(bool compress, const Item &single, const Item *items,
             hipLaunchKernelGGL((compress_kernel<TILE, false>), grid, block, 0, stream, single, items, tile_items,
                                cfg.total_tiles, cfg.d_total_tiles, ctrl, status, sc);
     } else {
-        if (items)
+        if (items && cfg.tile_off)
+            hipLaunchKernelGGL((decompress_kernel<TILE, true, true>), grid, block, 0, stream, single, items,
+                               tile_items, cfg.total_tiles, cfg.d_total_tiles, ctrl, status, cfg.tile_off, sc, 1u);
+        else if (items)
             hipLaunchKernelGGL((decompress_kernel<TILE, true>), grid, block, 0, stream, single, items, tile_items,
                                cfg.total_tiles, cfg.d_total_tiles, ctrl, status, (uint64_t *)nullptr, sc, 0u);
         else if (single.ntiles <= kSmallTiles)

@@ -71,6 +71,48 @@ def test_plan_refill_and_graph_replay(gpu, orc):
     plan.close()
 
 
+def test_decode_plan_refill_and_graph_replay(gpu, orc):
+    """One decompression plan over several batches (graph replays, items of
+    1 to 75 tiles, a corrupt stream in one batch): the offset phase runs over
+    every item's tiles and each item decodes to the oracle's volume; the
+    corrupt item is reported and the next batch decodes cleanly."""
+    import torch
+    vols_all = _frames(3, 30, sides=(8, 64, 136, 512, 1100))
+    params_all = [((1, 1, 1, 1), 1.0) if i % 3 else ((3, 5, 10, 15), 0.5) for i in range(len(vols_all))]
+    refs = [orc.compress(v, b, p)[1] for v, (b, p) in zip(vols_all, params_all)]
+    maxb = sum(gpu.num_blocks(v.shape) for v in vols_all)
+    dplan = gpu.Plan(gpu.DECOMPRESS, 30, maxb)
+    for rep, (lo, hi) in enumerate([(0, 10), (10, 30), (5, 12), (0, 30), (3, 4)]):
+        vols, params, streams = vols_all[lo:hi], params_all[lo:hi], refs[lo:hi]
+        src = [torch.from_numpy(r.copy()).cuda() for r in streams]
+        outs = [torch.full(v.shape, 7, dtype=torch.int16, device="cuda") for v in vols]
+        dplan.set_items([(o, v.shape, b, p, sb, sb.numel()) for o, v, (b, p), sb in zip(outs, vols, params, src)])
+        dplan.launch(graph=True)
+        assert dplan.status() == gpu.OK, rep
+        for o, v, (b, p), r in zip(outs, vols, params, streams):
+            _, refdec = orc.decompress(r, v.shape, b)
+            assert np.array_equal(o.cpu().numpy().view(np.uint16), refdec), (rep, v.shape)
+    # a header byte with rate 31 in one item: flagged, that item's bad block decodes to zeros
+    bad = refs[4].copy()
+    bad[0] = 0x1f
+    vols, params = vols_all[:6], params_all[:6]
+    src = [torch.from_numpy((bad if i == 4 else refs[i]).copy()).cuda() for i in range(6)]
+    outs = [torch.full(v.shape, 7, dtype=torch.int16, device="cuda") for v in vols]
+    dplan.set_items([(o, v.shape, b, p, sb, sb.numel()) for o, v, (b, p), sb in zip(outs, vols, params, src)])
+    dplan.launch(graph=True)
+    assert dplan.status() == gpu.ERR_CORRUPT
+    for i in (0, 1, 2, 3, 5):
+        _, refdec = orc.decompress(refs[i], vols[i].shape, params[i][0])
+        assert np.array_equal(outs[i].cpu().numpy().view(np.uint16), refdec)
+    src[4] = torch.from_numpy(refs[4].copy()).cuda()
+    dplan.set_items([(o, v.shape, b, p, sb, sb.numel()) for o, v, (b, p), sb in zip(outs, vols, params, src)])
+    dplan.launch(graph=True)
+    assert dplan.status() == gpu.OK
+    _, refdec = orc.decompress(refs[4], vols[4].shape, params[4][0])
+    assert np.array_equal(outs[4].cpu().numpy().view(np.uint16), refdec)
+    dplan.close()
+
+
 def test_plan_rejects_bad_items(gpu):
     import torch
     plan = gpu.Plan(gpu.COMPRESS, 2, 64)
