@@ -1171,8 +1171,9 @@ __device__ __forceinline__ uint64_t tile_offset(const uint64_t *tile_off, uint32
     return off_value(tile_off, u, off_word(tile_off, u), epoch, ctrl);
 }
 
-// First phase of a single-volume decode launch: the payload offset of every
-// tile, from the header bytes alone (1/128 of the volume).  Chunks of 128
+// First phase of a decode launch (a single volume, or a batch: below): the
+// payload offset of every tile, from the header bytes alone (1/128 of the
+// volume).  Chunks of 128
 // tiles come from their own counter, so every chunk is taken before any
 // workgroup starts decoding.  TILE/128 threads sum 8 16-byte header pieces
 // each of one tile (each load instruction of a wave reads 64 contiguous header
