@@ -17,8 +17,9 @@ served form (512^2..2048^2 CSSI frames, half lossless and half lossy, under a
 saturation, beside the host-scheduled engine and plain per-frame launches).
 
 Each mode also carries the end-to-end time of one blocking call (the stream size
-or status read back on the host, as the reference's CLI times its calls) and that
-rate in the reference's unit (input MiB per ms).
+or status read back on the host, as the reference's CLI times its calls; median
+and mean of `--steps` calls) and that rate in the reference's unit (input MiB per
+ms).
 
 Prints ONE JSON line.  `--gpus` above 1 is refused: this build runs on one GPU.
 """
@@ -95,15 +96,20 @@ def timed_loop(torch, fn, steps):
     return wall, sum(per) / len(per), per
 
 
-def e2e_ms(torch, fn, steps):
-    """Mean wall-clock ms of `steps` blocking calls (each returns after its stream
-    size or status has reached the host)."""
+def e2e_ms(torch, fn, steps, median=False):
+    """Wall-clock ms of `steps` blocking calls (each returns after its stream size
+    or status has reached the host): their mean, or with `median` the (median,
+    mean) of the per-call times (one stalled call moves the mean, not the median)."""
     fn()
     torch.cuda.synchronize()
+    per = []
     t0 = time.perf_counter()
     for _ in range(steps):
+        t = time.perf_counter()
         fn()
-    return (time.perf_counter() - t0) * 1e3 / steps
+        per.append((time.perf_counter() - t) * 1e3)
+    mean = (time.perf_counter() - t0) * 1e3 / steps
+    return (statistics.median(per), mean) if median else mean
 
 
 def profiled_traffic(dims):
@@ -149,9 +155,11 @@ def time_codec(torch, lscomp, d_in, dims, bins, pool, d_out, d_dec, d_size, args
     # end to end per blocking call, as the reference's CLI times it (lsCOMP_xpcs.cpp:144-174:
     # the call returns with the stream size on the host), and in the reference's unit
     # (input MiB per ms, i.e. 1.048576e9 B/s)
-    c_e2e = e2e_ms(torch, lambda: lscomp.compress(d_in, dims, bins, pool, out=d_out), args.steps)
-    d_e2e = e2e_ms(torch, lambda: lscomp.decompress(d_out, s_bytes, dims, bins, out=d_dec), args.steps)
+    c_e2e, c_e2e_mean = e2e_ms(torch, lambda: lscomp.compress(d_in, dims, bins, pool, out=d_out), args.steps, True)
+    d_e2e, d_e2e_mean = e2e_ms(torch, lambda: lscomp.decompress(d_out, s_bytes, dims, bins, out=d_dec), args.steps,
+                               True)
     res.update({"compress_e2e_ms": c_e2e, "decompress_e2e_ms": d_e2e,
+                "compress_e2e_mean_ms": c_e2e_mean, "decompress_e2e_mean_ms": d_e2e_mean,
                 "compress_e2e_ref_unit_GBps": in_bytes / 1048576.0 / c_e2e,
                 "decompress_e2e_ref_unit_GBps": in_bytes / 1048576.0 / d_e2e})
     if check_lossless:
