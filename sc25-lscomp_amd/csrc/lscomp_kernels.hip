@@ -1417,18 +1417,17 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // but the whole header section is at most 128 KB and stays in L2.
 constexpr uint32_t kSmallTiles = 256;
 
-template <int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_small_kernel(
-    const Item it, Ctrl *ctrl)
+template <class C, int TILE>
+__device__ __forceinline__ void decompress_small_tiles(const Item &it, Ctrl *ctrl)
 {
-    __shared__ TileShared<TILE> sh;
+    __shared__ TileShared<TILE, C::kMaxWords> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t t = blockIdx.x;
     const uint32_t b = t * (uint32_t)TILE + tid;
     const bool valid = b < it.nblocks;
     const uint32_t hdr = valid ? (uint32_t)it.s_in[b] : 0u;
     // payload words before this tile: at most 255 tiles of 512 blocks of 32
-    // words, so 32 bits hold every partial sum
+    // words (or 256 blocks of 62), so 32 bits hold every partial sum
     uint32_t before = 0;
     const uint32_t npieces = t * ((uint32_t)TILE / 16u);
     for (uint32_t i = tid; i < npieces; i += 4u * TILE) {
@@ -1449,23 +1448,30 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 #pragma unroll
     for (int w = 0; w < TILE / 64; w++) excl += sh.wsum[1][w];
     const uint64_t g0 = (uint64_t)it.hdr_bytes + 4ull * excl;
-    const bool fits = agg <= (uint32_t)TILE * 32u && g0 + 4ull * agg <= it.s_len;
+    const bool fits = agg <= (uint32_t)TILE * C::kMaxWords && g0 + 4ull * agg <= it.s_len;
     if (fits) load_payload<TILE>(it.s_in, it.s_len, g0, agg, sh.pay, tid);
     __syncthreads();
     if (valid) {
-        const bool bad = (hdr & 0x1fu) > 16u || !fits;
-        uint32_t D[32];
+        const bool bad = C::bad_header(hdr) || !fits;
+        uint32_t D[C::kRegs];
         if (!bad && (hdr & 0x1fu)) {
-            decode_block(&sh.pay[((uint32_t)(g0 & 15) >> 2) + off], hdr, it.dec.b, D);
+            C::decode(&sh.pay[((uint32_t)(g0 & 15) >> 2) + off], hdr, it, D);
         } else {
 #pragma unroll
-            for (int i = 0; i < 32; i++) D[i] = 0;
+            for (int i = 0; i < C::kRegs; i++) D[i] = 0;
         }
         uint32_t bx, by, bz;
         block_coords(it, b, bx, by, bz);
-        store_block<true>(it, bx, by, bz, D);
+        C::store(it, bx, by, bz, D);
         if (bad) __hip_atomic_fetch_or(&ctrl->error, ERRF_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void decompress_small_kernel(
+    const Item it, Ctrl *ctrl)
+{
+    decompress_small_tiles<CodecU16, TILE>(it, ctrl);
 }
 
 // Decompression of a single item with precomputed tile offsets, pipelined so
@@ -2163,6 +2169,13 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void 
     decompress_pipe_tiles<CodecU32, TILE>(it, ctrl, status, tile_off, sc, phase);
 }
 
+template <int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void decompress_u32_small_kernel(
+    const Item it, Ctrl *ctrl)
+{
+    decompress_small_tiles<CodecU32, TILE>(it, ctrl);
+}
+
 hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,
                       hipStream_t stream)
 {
@@ -2170,6 +2183,8 @@ hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, 
     const dim3 grid(cfg.grid), block(T);
     if (compress) {
         hipLaunchKernelGGL((compress_u32_kernel<T>), grid, block, 0, stream, it, ctrl, status, cfg.sched);
+    } else if (it.ntiles <= kSmallTiles) {
+        hipLaunchKernelGGL((decompress_u32_small_kernel<T>), dim3(it.ntiles), block, 0, stream, it, ctrl);
     } else {
         if (!cfg.tile_off) return hipErrorInvalidValue;
         const uint32_t phase = cfg.off_mode == OFF_PHASE ? 1u : 0u;
