@@ -117,7 +117,13 @@ struct DevLoop {
     lsc::Sched sc{};
     bool running = false;
     uint64_t wr = 0;                   // next sequence number to publish
-    std::map<uint64_t, Req> inflight;  // published and not yet complete, by sequence number
+    // Published requests not yet complete: sequence number q lives in slot
+    // q % kSrvRing while live[] is set; lo is the oldest live one (or wr).  A
+    // reaping pass scans [lo, wr) in order over two flat arrays (a node-based
+    // map scanned at a full ring of 4096 cost most of the host thread's time).
+    std::vector<Req> req;
+    std::vector<uint8_t> live;
+    uint64_t lo = 0, nlive = 0;
     uint64_t instances = 0;
 };
 
@@ -277,20 +283,20 @@ void dev_reap(lscomp_server s)
 {
     DevLoop &d = s->dl;
     bool any = false;
-    for (auto it = d.inflight.begin(); it != d.inflight.end();) {
-        const uint64_t seq = it->first;
-        lsc::SrvComp &c = d.comp[seq % lsc::kSrvRing];
-        if (load_u64(&c.seq) != seq + 1) {
-            ++it;
-            continue;
-        }
+    for (uint64_t seq = d.lo; seq < d.wr && d.nlive; seq++) {
+        const size_t k = seq % lsc::kSrvRing;
+        if (!d.live[k]) continue;
+        lsc::SrvComp &c = d.comp[k];
+        if (load_u64(&c.seq) != seq + 1) continue;
         const uint64_t bytes = load_u64(&c.bytes);
         const uint32_t f = load_u32(&c.status);
         const int st = (f & lsc::SRV_TIMEOUT) ? LSCOMP_ERR_INTERNAL : LSCOMP_OK;   // capacity: from the size
-        complete(s, it->second, bytes, st, now_s());
-        it = d.inflight.erase(it);
+        complete(s, d.req[k], bytes, st, now_s());
+        d.live[k] = 0;
+        d.nlive--;
         any = true;
     }
+    while (d.lo < d.wr && !d.live[d.lo % lsc::kSrvRing]) d.lo++;
     if (any) s->cv_done.notify_all();
 }
 
@@ -299,7 +305,7 @@ void dev_admit(lscomp_server s)
     DevLoop &d = s->dl;
     while (!s->pending.empty()) {
         // the ring slot of sequence number wr is free once wr - kSrvRing has completed
-        if (!d.inflight.empty() && d.wr - d.inflight.begin()->first >= lsc::kSrvRing) break;
+        if (d.nlive && d.wr - d.lo >= lsc::kSrvRing) break;
         Req &q = s->pending.front();
         if (!admit_one(s, q)) break;
         q.item.vol_in = q.r.d_frame;
@@ -308,7 +314,10 @@ void dev_admit(lscomp_server s)
         q.item.seq = d.wr;
         q.t_launch = now_s();   // handed to the running loop
         std::memcpy(&d.ring[d.wr % lsc::kSrvRing].item, &q.item, sizeof(lsc::Item));
-        d.inflight.emplace(d.wr, q);
+        if (!d.nlive) d.lo = d.wr;
+        d.req[d.wr % lsc::kSrvRing] = q;
+        d.live[d.wr % lsc::kSrvRing] = 1;
+        d.nlive++;
         d.wr++;
         store_u64(&d.hc->wr, d.wr);   // release: the slot is written before it is published
         s->pending.pop_front();
@@ -339,8 +348,15 @@ int dev_launch(lscomp_server s)
 void dev_fail_all(lscomp_server s, int st)
 {
     const double t = now_s();
-    for (auto &kv : s->dl.inflight) complete(s, kv.second, 0, st, t);
-    s->dl.inflight.clear();
+    DevLoop &d = s->dl;
+    for (uint64_t seq = d.lo; seq < d.wr; seq++) {
+        const size_t k = seq % lsc::kSrvRing;
+        if (!d.live[k]) continue;
+        complete(s, d.req[k], 0, st, t);
+        d.live[k] = 0;
+    }
+    d.nlive = 0;
+    d.lo = d.wr;
     s->cv_done.notify_all();
 }
 
@@ -355,7 +371,7 @@ void worker_main(lscomp_server s)
             dev_admit(s);
             const int st = dev_launch(s);
             if (st != LSCOMP_OK) dev_fail_all(s, st);
-            busy = !s->dl.inflight.empty() || s->dl.running;
+            busy = s->dl.nlive != 0 || s->dl.running;
         } else {
             host_retire(s);
             host_admit(s);
@@ -380,6 +396,8 @@ int dev_create(lscomp_server s, uint32_t idle_us)
 {
     using namespace lsc;
     DevLoop &d = s->dl;
+    d.req.resize(lsc::kSrvRing);
+    d.live.assign(lsc::kSrvRing, 0);
     const unsigned pf = hipHostMallocCoherent | hipHostMallocMapped;
     if (hipHostMalloc((void **)&d.ring, sizeof(SrvReq) * kSrvRing, pf) != hipSuccess ||
         hipHostMalloc((void **)&d.comp, sizeof(SrvComp) * kSrvRing, pf) != hipSuccess ||
