@@ -319,7 +319,7 @@ int prepare_compress_u32(const uint32_t *d_in, uint8_t *d_out, size_t cap, lscom
 int prepare_decompress_u32(uint32_t *d_out, const uint8_t *d_in, size_t in_bytes, lscomp_dim3 dims,
                            lscomp_bins4 bins, Item &it)
 {
-    int st = make_item(dims, bins, 0.0f, it, kTileU32);
+    int st = make_item(dims, bins, 0.0f, it, kTileU32Dec);
     if (st) return st;
     if (!d_in || !d_out || ((uintptr_t)d_out & 3) || ((uintptr_t)d_in & 3)) return LSCOMP_ERR_INVALID_ARG;
     if (in_bytes < it.hdr_bytes) return LSCOMP_ERR_CORRUPT;
@@ -344,7 +344,7 @@ int launch_item_u32(Scratch *s, Item &it, bool compress)
         occ[compress].store(o);
     }
     LaunchCfg cfg{};
-    cfg.tile = kTileU32;
+    cfg.tile = tile_u32(compress);
     cfg.total_tiles = it.ntiles;
     uint64_t g = (uint64_t)(num_cus() > 0 ? num_cus() : 1) * (uint64_t)o;
     if (g > it.ntiles) g = it.ntiles;
@@ -352,7 +352,7 @@ int launch_item_u32(Scratch *s, Item &it, bool compress)
     // the direction's counters; the window of 64 words timed best at this tile
     const uint32_t nc = load_shape(compress ? 0 : 1).nc;
     cfg.sched.nc = nc < cfg.grid ? (nc ? nc : 1u) : cfg.grid;
-    cfg.sched.win = default_window(kTileU32);
+    cfg.sched.win = default_window(cfg.tile);
     if (compress) {
         if (!it.result) it.result = &s->ctrl->result;
     } else {

@@ -2179,18 +2179,19 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2))) void 
 hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,
                       hipStream_t stream)
 {
-    constexpr int T = (int)kTileU32;
-    const dim3 grid(cfg.grid), block(T);
+    constexpr int T = (int)kTileU32, TD = (int)kTileU32Dec;
+    if (cfg.tile != (uint32_t)(compress ? T : TD)) return hipErrorInvalidValue;
+    const dim3 grid(cfg.grid);
     if (compress) {
-        hipLaunchKernelGGL((compress_u32_kernel<T>), grid, block, 0, stream, it, ctrl, status, cfg.sched);
+        hipLaunchKernelGGL((compress_u32_kernel<T>), grid, dim3(T), 0, stream, it, ctrl, status, cfg.sched);
     } else if (it.ntiles <= kSmallTiles) {
-        hipLaunchKernelGGL((decompress_u32_small_kernel<T>), dim3(it.ntiles), block, 0, stream, it, ctrl);
+        hipLaunchKernelGGL((decompress_u32_small_kernel<TD>), dim3(it.ntiles), dim3(TD), 0, stream, it, ctrl);
     } else {
         if (!cfg.tile_off) return hipErrorInvalidValue;
         const uint32_t phase = cfg.off_mode == OFF_PHASE ? 1u : 0u;
-        if (!phase) launch_offsets<T>(it, cfg, ctrl, status, stream);
-        hipLaunchKernelGGL((decompress_u32_pipe_kernel<T>), grid, block, 0, stream, it, ctrl, status, cfg.tile_off,
-                           cfg.sched, phase);
+        if (!phase) launch_offsets<TD>(it, cfg, ctrl, status, stream);
+        hipLaunchKernelGGL((decompress_u32_pipe_kernel<TD>), grid, dim3(TD), 0, stream, it, ctrl, status,
+                           cfg.tile_off, cfg.sched, phase);
     }
     return hipGetLastError();
 }
@@ -2253,11 +2254,11 @@ int workgroups_per_cu(bool compress, bool table, int tile)
 
 int workgroups_per_cu_u32(bool compress)
 {
-    constexpr int T = (int)kTileU32;
+    constexpr int T = (int)kTileU32, TD = (int)kTileU32Dec;
     int per_cu = 0;
     const hipError_t e = compress
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, compress_u32_kernel<T>, T, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_u32_pipe_kernel<T>, T, 0);
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_u32_pipe_kernel<TD>, TD, 0);
     return (e == hipSuccess && per_cu > 0) ? per_cu : 1;
 }
 

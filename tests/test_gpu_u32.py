@@ -154,10 +154,10 @@ def test_u32_many_tiles_per_workgroup(gpu, orc, kind):
     check_u32(gpu, orc, np.ascontiguousarray(vol[:4]), (1, 1, 1, 1), 1.0)
 
 
-@pytest.mark.parametrize("shape", [(1, 2048, 2048), (1, 2048, 2056)], ids=["256tiles", "257tiles"])
+@pytest.mark.parametrize("shape", [(2, 2048, 2048), (2, 2048, 2056)], ids=["256tiles", "257tiles"])
 def test_u32_small_volume_decode_limit(gpu, orc, shape):
-    """uint32 volumes of at most 256 tiles (of 256 blocks) decode with one
-    workgroup per tile; 257 tiles take the offset-phase launch.  Both sides of
-    the limit equal the oracle."""
+    """uint32 volumes of at most 256 decode tiles (of 512 blocks) decode with
+    one workgroup per tile; 257 tiles take the offset-phase launch.  Both sides
+    of the limit equal the oracle."""
     for bins, p in (((1, 1, 1, 1), 1.0), ((3, 5, 10, 15), 0.5)):
         check_u32(gpu, orc, synth_u32("counts", shape, 9), bins, p)
