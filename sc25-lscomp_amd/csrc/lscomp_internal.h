@@ -87,6 +87,15 @@ struct Item {
     uint64_t seq;             // served form: the request's sequence number
 };
 
+// A single-volume decode's stream holds 2 or more payload bytes per block on
+// average: the pipelined decoder overlaps its payload loads with decoding
+// (below that the decoder that stages each tile's payload in its own
+// iteration was faster).
+inline __host__ __device__ bool dense_stream(const Item &it)
+{
+    return it.s_len >= (uint64_t)it.hdr_bytes + 2ull * it.nblocks;
+}
+
 // Launch shape of one direction: the stream never depends on it.
 struct Shape {
     uint32_t tile;      // blocks per tile (= threads per workgroup): 128, 256 or 512
@@ -132,6 +141,13 @@ hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t value, hipStream_t 
 // 4-wave ones on tiles of 256, and 6-11 % slower compressing); decompression
 // always computes tile offsets first (cfg.tile_off must be set).
 constexpr uint32_t kTileU32 = 256;
+// Decode geometry of a single uint16 volume.  At most kSmallTiles tiles: one
+// workgroup per tile that sums the header bytes before it.  A payload-dense
+// stream (dense_stream) at the default and largest launch tile (512) with the
+// offset phase decodes in the pipelined decoder on tiles of kPipeTile blocks,
+// one 16-wave workgroup per CU.
+constexpr uint32_t kSmallTiles = 256;
+constexpr uint32_t kPipeTile = 1024;
 constexpr uint32_t kTileU32Dec = 512;
 constexpr uint32_t tile_u32(bool compress) { return compress ? kTileU32 : kTileU32Dec; }
 hipError_t launch_u32(bool compress, const Item &it, LaunchCfg cfg, Ctrl *ctrl, uint64_t *status,

@@ -1415,8 +1415,6 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) void 
 // offset phase's chunk ticket, header pass, look-back and tile ids ahead of
 // the decode.  The header bytes before a tile are read once per later tile,
 // but the whole header section is at most 128 KB and stays in L2.
-constexpr uint32_t kSmallTiles = 256;
-
 template <class C, int TILE>
 __device__ __forceinline__ void decompress_small_tiles(const Item &it, Ctrl *ctrl)
 {
@@ -1759,7 +1757,7 @@ static void launch_offsets(const Item &it, const LaunchCfg &cfg, Ctrl *ctrl, uin
 // a tile's payload load is short and the stores dominate).
 static bool use_pipe_decode(const Item &it)
 {
-    return it.s_len >= (uint64_t)it.hdr_bytes + 2ull * it.nblocks;
+    return dense_stream(it);
 }
 
 template <int TILE>
@@ -2203,6 +2201,12 @@ static hipError_t launch(bool compress, const Item &single, const Item *items, c
     case 128: return launch_t<128>(compress, single, items, tile_items, cfg, ctrl, status, stream);
     case 256: return launch_t<256>(compress, single, items, tile_items, cfg, ctrl, status, stream);
     case 512: return launch_t<512>(compress, single, items, tile_items, cfg, ctrl, status, stream);
+    case kPipeTile:   // the pipelined decode of one dense volume with the offset phase only
+        if (compress || items || !cfg.tile_off || cfg.off_mode != OFF_PHASE || !dense_stream(single))
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((decompress_pipe_kernel<kPipeTile>), dim3(cfg.grid), dim3(kPipeTile), 0, stream, single,
+                           ctrl, status, cfg.tile_off, cfg.sched, 1u);
+        return hipGetLastError();
     default: return hipErrorInvalidValue;
     }
 }
@@ -2248,6 +2252,12 @@ int workgroups_per_cu(bool compress, bool table, int tile)
     case 128: return per_cu_t<128>(compress, table);
     case 256: return per_cu_t<256>(compress, table);
     case 512: return per_cu_t<512>(compress, table);
+    case kPipeTile: {
+        int per_cu = 0;
+        const hipError_t e =
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decompress_pipe_kernel<kPipeTile>, kPipeTile, 0);
+        return (!compress && !table && e == hipSuccess && per_cu > 0) ? per_cu : 1;
+    }
     default: return 1;
     }
 }
